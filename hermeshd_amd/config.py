@@ -1,0 +1,166 @@
+"""Run configuration: the runtime equivalent of HERMESHD's compile-time
+``input.f90`` / ``params.f90`` switches.
+
+Reference: src/input.f90:1-108, src/params.f90:42-129, tests/*/input_*.f90.
+The field and mode layout of the state follows src/params.f90:42-47,163 and
+src/initialize.f90:126-194.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+from typing import Sequence, Tuple
+
+BC_NAMES = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3}
+RIEMANN_NAMES = {"central": 0, "hllc": 1, "roe": 2}
+INTEGRATOR_NAMES = {"heun": 2, "shu-osher": 3}
+VALID_BASIS = {(4, 2), (8, 2), (10, 3)}
+
+# field indices (0-based) of src/params.f90:42-47
+RH, MX, MY, MZ, EN, PXX, PYY, PZZ, PXY, PXZ, PYZ = range(11)
+
+
+def reference_T_floor(TK: float = 300.0, mu: float = 18.0) -> float:
+    """T_floor of src/params.f90:78-97: (TK*eV_per_K)/te0 in code units."""
+    eV_per_K = 8.61728e-5
+    L0, t0, n0 = 1.0, 1.0, 2.5e25
+    v0 = L0 / t0
+    p0 = mu * 1.67e-27 * n0 * v0 ** 2
+    te0 = p0 / n0 / 1.6e-19
+    return (TK * eV_per_K) / te0
+
+
+def default_cflm(iquad: int) -> float:
+    """set_cflm (src/initialize.f90:104-122)."""
+    return {2: 0.14, 3: 0.08, 4: 0.05}[iquad]
+
+
+@dataclass
+class HDConfig:
+    nx: int
+    ny: int
+    nz: int
+    nbasis: int = 4
+    iquad: int = 2
+    nQ: int = 5
+    ivis: int = 0
+    ieos: int = 1
+    riemann: str = "hllc"
+    limiter: bool = True
+    integrator: str = "heun"
+    bc: Tuple[str, str, str, str, str, str] = ("periodic",) * 6
+    lx: float = 1.0
+    ly: float = 1.0
+    lz: float = 1.0
+    gamma: float = 1.4
+    rh_floor: float = 5.0e-6
+    T_floor: float = field(default_factory=reference_T_floor)
+    coll: float = 0.0
+    colvis: float = 0.0
+    cflm: float | None = None
+    body_force: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    noise: bool = False
+    eta_sd: float = 0.0
+    zeta_sd: float = 0.0
+    seed: int = 1234
+    realization: int = 0
+
+    def __post_init__(self):
+        if (self.nbasis, self.iquad) not in VALID_BASIS:
+            raise ValueError(f"(nbasis, iquad) = {(self.nbasis, self.iquad)} not in {sorted(VALID_BASIS)}")
+        if self.nQ not in (5, 11):
+            raise ValueError("nQ must be 5 (hydro fields) or 11 (hydro + stress, reference layout)")
+        if self.ivis not in (0, 1):
+            raise ValueError("ivis must be 0 (explicit) or 1 (semi-implicit stress relaxation)")
+        if self.ivis == 1 and self.nQ != 11:
+            raise ValueError("ivis=1 needs the stress fields (nQ=11)")
+        if self.noise and self.nQ != 11:
+            raise ValueError("stochastic stress needs nQ=11")
+        if self.ieos != 1:
+            raise ValueError("only ieos=1 (ideal gas) is supported")
+        if self.riemann not in RIEMANN_NAMES:
+            raise ValueError(f"riemann must be one of {list(RIEMANN_NAMES)}")
+        if self.integrator not in INTEGRATOR_NAMES:
+            raise ValueError(f"integrator must be one of {list(INTEGRATOR_NAMES)}")
+        bc = tuple(self.bc)
+        if len(bc) != 6 or any(b not in BC_NAMES for b in bc):
+            raise ValueError(f"bc must be 6 names from {list(BC_NAMES)}")
+        for d in range(3):
+            if (bc[2 * d] == "periodic") != (bc[2 * d + 1] == "periodic"):
+                raise ValueError("a periodic axis must be periodic on both sides")
+        self.bc = bc
+        if min(self.nx, self.ny, self.nz) < 1:
+            raise ValueError("grid extents must be positive")
+        if self.cflm is None:
+            self.cflm = default_cflm(self.iquad)
+
+    # ------------------------------------------------------------------
+    @property
+    def shape(self) -> Tuple[int, int, int, int, int]:
+        """Torch/C shape (nbasis, nQ, nz, ny, nx) == Fortran Q(nx,ny,nz,nQ,nbasis)."""
+        return (self.nbasis, self.nQ, self.nz, self.ny, self.nx)
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def dxi(self) -> float:
+        return self.nx / (self.lx / 2.0 - (-(self.lx / 2.0)))
+
+    @property
+    def P_floor(self) -> float:
+        return self.T_floor * self.rh_floor
+
+    def replace(self, **kw) -> "HDConfig":
+        return dataclasses.replace(self, **kw)
+
+    def as_ints(self) -> dict:
+        """Numeric form shared by the C-ABI and the oracle."""
+        return dict(
+            nx=self.nx, ny=self.ny, nz=self.nz, nbasis=self.nbasis, iquad=self.iquad, nQ=self.nQ,
+            ivis=self.ivis, ieos=self.ieos, riemann=RIEMANN_NAMES[self.riemann],
+            limiter=int(bool(self.limiter)), integrator=INTEGRATOR_NAMES[self.integrator],
+            bc=[BC_NAMES[b] for b in self.bc], noise=int(bool(self.noise)),
+            lx=self.lx, ly=self.ly, lz=self.lz, gamma=self.gamma, rh_floor=self.rh_floor,
+            T_floor=self.T_floor, coll=self.coll, colvis=self.colvis, cflm=self.cflm,
+            body_force=list(self.body_force), eta_sd=self.eta_sd, zeta_sd=self.zeta_sd,
+            seed=int(self.seed), realization=int(self.realization),
+        )
+
+
+# ----------------------------------------------------------------------
+# The five workloads, as concrete configurations.
+# ----------------------------------------------------------------------
+def sod_1d(nx: int = 256) -> HDConfig:
+    """1D Sod shock tube: 256 cells on [0,1], linear basis, HLLC, limiter,
+    SSP-RK2 at CFL 0.3, outflow ends.  The floors are set well below the
+    states (rh_floor 5e-6, P_floor 5e-10) since this workload uses p ~ 1."""
+    dx = 1.0 / nx
+    return HDConfig(nx=nx, ny=1, nz=1, nbasis=4, iquad=2, nQ=5, riemann="hllc", limiter=True,
+                    integrator="heun", bc=("outflow", "outflow", "periodic", "periodic", "periodic", "periodic"),
+                    lx=1.0, ly=dx, lz=dx, gamma=1.4, T_floor=1e-4, cflm=0.3)
+
+
+def vortex_3d(nx: int = 128, ny: int = 128, nz: int = 32) -> HDConfig:
+    """Isentropic vortex extruded in z: periodic box of width 10 in x with cubic
+    cells ([0,10]^2 x [0,2.5] at 128x128x32), complete-quadratic basis,
+    3-point Gauss, HLLC, SSP-RK3.  The reference's dt uses dxi only
+    (src/hermeshd.f90:174), so cells must stay cubic for the step to be
+    stable in y and z."""
+    h = 10.0 / nx
+    return HDConfig(nx=nx, ny=ny, nz=nz, nbasis=10, iquad=3, nQ=5, riemann="hllc", limiter=True,
+                    integrator="shu-osher", bc=("periodic",) * 6, lx=10.0, ly=h * ny, lz=h * nz,
+                    gamma=1.4, T_floor=1e-4)
+
+
+def poiseuille_2d(nx: int = 64, ny: int = 128, vis: float = 0.01) -> HDConfig:
+    """10-moment viscous relaxation channel: rho = T = 1, coll = rho*T/vis,
+    colvis = coll*vis (src/initialcon.f90:258-259), body force 1e-3 in x,
+    no-slip walls in y, periodic in x, limiter off."""
+    coll = 1.0 * 1.0 / vis
+    return HDConfig(nx=nx, ny=ny, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, riemann="hllc", limiter=False,
+                    integrator="heun", bc=("periodic", "periodic", "noslip", "noslip", "periodic", "periodic"),
+                    lx=float(nx), ly=float(ny), lz=1.0, gamma=5.0 / 3.0, T_floor=1.0, coll=coll,
+                    colvis=coll * vis, body_force=(1e-3, 0.0, 0.0))

@@ -1,0 +1,371 @@
+// hd_capi.hip -- C ABI of the HERMESHD hot path on MI355X (gfx950).
+//
+// A context owns the basis tables, geometry and scratch of one configuration
+// (the reference keeps these as Fortran module globals; src/params.f90,
+// src/basis_funcs.f90, src/boundary.f90, src/flux.f90:11-18).  All entry
+// points are stream-ordered and return an error code; none aborts.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hermeshd_amd.h"
+#include "hd_generic.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(HD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+}  // namespace
+
+struct hd_ctx {
+    hd_config cfg;
+    int q, nb, nQ, nface;
+    long N;
+    hd::Grid grid;
+    hd::Phys ph;
+    hd::LimParams lp;
+    hd::Tables htab;
+    hd::Tables* dtab = nullptr;
+    double* fl[3] = {nullptr, nullptr, nullptr};  // face fluxes per direction
+    double* partial = nullptr;                    // wave-speed partial maxima
+    int npart = 0;
+    int* err = nullptr;
+    unsigned long long* bad = nullptr;
+    double* scratch_dt = nullptr;
+    int64_t bytes = 0;
+    int device = 0;
+    // stage timing (hd_timing_enable): event pairs recorded around each stage
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+using namespace hd;
+
+// ---------------------------------------------------------------------------
+// template dispatch over (nbasis, iquad) x nQ x Riemann solver
+// ---------------------------------------------------------------------------
+struct Ops {
+    void (*limiter)(hd_ctx*, double*, hipStream_t);
+    void (*faces)(hd_ctx*, const double*, hipStream_t);
+    void (*cell)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
+};
+
+inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int NB, int QP, int NQ, int RM>
+struct Impl {
+    static void limiter(hd_ctx* c, double* q, hipStream_t s) {
+        hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
+    }
+    static void faces(hd_ctx* c, const double* q, hipStream_t s) {
+        const Grid& g = c->grid;
+        long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz, nfz = (long)g.nx * g.ny * (g.nz + 1);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph);
+    }
+    static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
+                           c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
+    }
+    static Ops ops() { return Ops{&limiter, &faces, &cell}; }
+};
+
+template <int NB, int QP, int NQ>
+bool pick_rm(int rm, Ops* o) {
+    switch (rm) {
+        case RIEMANN_CENTRAL: *o = Impl<NB, QP, NQ, RIEMANN_CENTRAL>::ops(); return true;
+        case RIEMANN_HLLC: *o = Impl<NB, QP, NQ, RIEMANN_HLLC>::ops(); return true;
+        case RIEMANN_ROE: *o = Impl<NB, QP, NQ, RIEMANN_ROE>::ops(); return true;
+    }
+    return false;
+}
+
+template <int NB, int QP>
+bool pick_nq(int nq, int rm, Ops* o) {
+    if (nq == 5) return pick_rm<NB, QP, 5>(rm, o);
+    if (nq == 11) return pick_rm<NB, QP, 11>(rm, o);
+    return false;
+}
+
+bool pick_ops(const hd_ctx* c, Ops* o) {
+    if (c->nb == 4 && c->q == 2) return pick_nq<4, 2>(c->nQ, c->cfg.riemann, o);
+    if (c->nb == 8 && c->q == 2) return pick_nq<8, 2>(c->nQ, c->cfg.riemann, o);
+    if (c->nb == 10 && c->q == 3) return pick_nq<10, 3>(c->nQ, c->cfg.riemann, o);
+    return false;
+}
+
+int validate(const hd_config* c) {
+    if (!c) return fail(HD_ERR_INVALID, "config is NULL");
+    if (c->nx < 1 || c->ny < 1 || c->nz < 1) return fail(HD_ERR_INVALID, "grid extents must be >= 1");
+    const bool ok = (c->nbasis == 4 && c->iquad == 2) || (c->nbasis == 8 && c->iquad == 2) ||
+                    (c->nbasis == 10 && c->iquad == 3);
+    if (!ok) return fail(HD_ERR_INVALID, "(nbasis, iquad) must be (4,2), (8,2) or (10,3)");
+    if (c->nQ != 5 && c->nQ != 11) return fail(HD_ERR_INVALID, "nQ must be 5 or 11");
+    if (c->ivis != 0 && c->ivis != 1) return fail(HD_ERR_INVALID, "ivis must be 0 or 1");
+    if (c->ivis == 1 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=1 needs nQ=11");
+    if (c->ieos != 1) return fail(HD_ERR_INVALID, "only ieos=1 (ideal gas) is supported");
+    if (c->riemann < 0 || c->riemann > 2) return fail(HD_ERR_INVALID, "riemann must be 0, 1 or 2");
+    if (c->integrator != 2 && c->integrator != 3) return fail(HD_ERR_INVALID, "integrator must be 2 or 3");
+    for (int d = 0; d < 3; d++) {
+        for (int s = 0; s < 2; s++)
+            if (c->bc[2 * d + s] < 0 || c->bc[2 * d + s] > 3) return fail(HD_ERR_INVALID, "unknown boundary kind");
+        if ((c->bc[2 * d] == HD_BC_PERIODIC) != (c->bc[2 * d + 1] == HD_BC_PERIODIC))
+            return fail(HD_ERR_INVALID, "a periodic axis must be periodic on both sides");
+    }
+    if (c->noise) return fail(HD_ERR_UNSUPPORTED, "stochastic stress is served by the ensemble entry points");
+    if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) return fail(HD_ERR_INVALID, "domain lengths must be positive");
+    if (!(c->gamma > 1.0)) return fail(HD_ERR_INVALID, "gamma must exceed 1");
+    return HD_OK;
+}
+
+int alloc(hd_ctx* c, void** p, size_t bytes) {
+    HIPCHK(hipMalloc(p, bytes));
+    c->bytes += (int64_t)bytes;
+    return HD_OK;
+}
+
+StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int mode) {
+    StageParams sp;
+    sp.dt = dt_dev;
+    sp.coll = c->cfg.coll;
+    sp.ivis = c->cfg.ivis;
+    sp.combine = combine;
+    sp.mode = mode;
+    for (int d = 0; d < 3; d++) sp.bf[d] = c->cfg.body_force[d];
+    sp.err = c->err;
+    sp.bad = c->bad;
+    return sp;
+}
+
+hipEvent_t next_event(hd_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// one Euler stage with optional fused SSP combination
+int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* dt_dev, int combine, hipStream_t s) {
+    Ops o;
+    if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        e0 = next_event(c); e1 = next_event(c);
+        if (!e0 || !e1) return fail(HD_ERR_HIP, "hipEventCreate failed");
+        HIPCHK(hipEventRecord(e0, s));
+    }
+    if (c->cfg.limiter) o.limiter(c, qin, s);
+    o.faces(c, qin, s);
+    StageParams sp = stage_params(c, dt_dev, combine, 0);
+    o.cell(c, qin, qn, qout, sp, s);
+    HIPCHK(hipGetLastError());
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, s));
+        c->ev_pending.emplace_back(e0, e1);
+    }
+    return HD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hd_last_error(void) { return g_last_error.c_str(); }
+
+int hd_create(const hd_config* cfg, hd_ctx** out) {
+    if (!out) return fail(HD_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    hd_ctx* c = new (std::nothrow) hd_ctx();
+    if (!c) return fail(HD_ERR_INVALID, "out of host memory");
+    c->cfg = *cfg;
+    c->q = cfg->iquad; c->nb = cfg->nbasis; c->nQ = cfg->nQ; c->nface = c->q * c->q;
+    c->N = (long)cfg->nx * cfg->ny * cfg->nz;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(HD_ERR_HIP, "no HIP device"); }
+    Grid& g = c->grid;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz; g.nQ = cfg->nQ; g.N = c->N;
+    // init_spatial_params (src/initialize.f90:298-327) with one rank
+    g.dxi = (double)cfg->nx / (cfg->lx / 2.0 - (-(cfg->lx / 2.0)));
+    g.dyi = (double)cfg->ny / (cfg->ly / 2.0 - (-(cfg->ly / 2.0)));
+    g.dzi = (double)cfg->nz / (cfg->lz / 2.0 - (-(cfg->lz / 2.0)));
+    for (int s = 0; s < 6; s++) g.bc[s] = cfg->bc[s];
+    Phys& ph = c->ph;
+    ph.gamma = cfg->gamma; ph.gm1 = cfg->gamma - 1.;
+    ph.P_floor = cfg->T_floor * cfg->rh_floor; ph.rh_floor = cfg->rh_floor;
+    ph.colvis = cfg->colvis; ph.c2d3cv = (2. / 3.) * cfg->colvis; ph.c4d3cv = (4. / 3.) * cfg->colvis;
+    c->lp.rh_floor = cfg->rh_floor; c->lp.epsP = cfg->rh_floor * cfg->T_floor; c->lp.gm1 = ph.gm1;
+    build_tables(c->nb, c->q, &c->htab);
+    int r = HD_OK;
+    const size_t nf = (size_t)c->nface * c->nQ;
+    const size_t nfx = (size_t)(g.nx + 1) * g.ny * g.nz, nfy = (size_t)g.nx * (g.ny + 1) * g.nz,
+                 nfz = (size_t)g.nx * g.ny * (g.nz + 1);
+    c->npart = 1024;
+    if ((r = alloc(c, (void**)&c->dtab, sizeof(Tables))) ||
+        (r = alloc(c, (void**)&c->fl[0], nf * nfx * sizeof(double))) ||
+        (r = alloc(c, (void**)&c->fl[1], nf * nfy * sizeof(double))) ||
+        (r = alloc(c, (void**)&c->fl[2], nf * nfz * sizeof(double))) ||
+        (r = alloc(c, (void**)&c->partial, c->npart * sizeof(double))) ||
+        (r = alloc(c, (void**)&c->err, sizeof(int))) ||
+        (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
+        (r = alloc(c, (void**)&c->scratch_dt, sizeof(double)))) {
+        std::string msg = g_last_error;
+        hd_destroy(c);
+        return fail(r, msg);
+    }
+    if (hipMemcpy(c->dtab, &c->htab, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->err, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(c->bad, 0xff, sizeof(unsigned long long)) != hipSuccess) {
+        hd_destroy(c);
+        return fail(HD_ERR_HIP, "initialising context buffers failed");
+    }
+    *out = c;
+    return HD_OK;
+}
+
+int hd_destroy(hd_ctx* c) {
+    if (!c) return HD_OK;
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    void* ptrs[] = {c->dtab, c->fl[0], c->fl[1], c->fl[2], c->partial, c->err, c->bad, c->scratch_dt};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+    return HD_OK;
+}
+
+int64_t hd_device_bytes(const hd_ctx* c) { return c ? c->bytes : 0; }
+
+int hd_timing_enable(hd_ctx* c, int enable) {
+    if (!c) return fail(HD_ERR_INVALID, "NULL context");
+    c->timing = enable != 0;
+    return HD_OK;
+}
+
+int hd_timing_read(hd_ctx* c, double* total_ms, int64_t* stages) {
+    if (!c || !total_ms || !stages) return fail(HD_ERR_INVALID, "NULL argument");
+    double sum = 0.;
+    for (auto& pr : c->ev_pending) {
+        HIPCHK(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *stages = (int64_t)c->ev_pending.size();
+    c->ev_pending.clear();
+    c->ev_used = 0;
+    return HD_OK;
+}
+
+int hd_limit(hd_ctx* c, double* q, void* stream) {
+    if (!c || !q) return fail(HD_ERR_INVALID, "NULL argument");
+    Ops o;
+    if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    o.limiter(c, q, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_rhs(hd_ctx* c, const double* q, double* rhs, void* stream) {
+    if (!c || !q || !rhs) return fail(HD_ERR_INVALID, "NULL argument");
+    Ops o;
+    if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    hipStream_t s = (hipStream_t)stream;
+    o.faces(c, q, s);
+    // dt is not used in rhs mode; point at the context's scratch scalar
+    StageParams sp = stage_params(c, c->scratch_dt, 0, 1);
+    o.cell(c, q, q, rhs, sp, s);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_euler_step(hd_ctx* c, double* q_in, double* q_out, const double* dt_dev, void* stream) {
+    if (!c || !q_in || !q_out || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q_in == q_out) return fail(HD_ERR_INVALID, "q_in and q_out must differ");
+    return stage(c, q_in, q_in, q_out, dt_dev, 0, (hipStream_t)stream);
+}
+
+int hd_min_dt(hd_ctx* c, const double* q, double* dt_dev, void* stream) {
+    if (!c || !q || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = (int)std::min<long>(c->npart, (c->N + 255) / 256);
+    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q, c->grid, c->cfg.gamma, 1.01 * c->cfg.rh_floor,
+                       c->partial);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q, c->grid, c->cfg.gamma, c->partial, nblk, c->cfg.cflm,
+                       dt_dev, (double*)nullptr);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_dev, void* stream) {
+    (void)q2;
+    if (!c || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q_io == q1) return fail(HD_ERR_INVALID, "q_io and q1 must differ");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (c->cfg.integrator == 3) {
+        // RK3 (src/integrator.f90:74-86): Q1 = E(Q0); Q1 = 3/4 Q0 + 1/4 E(Q1); Q0 = 1/3 Q0 + 2/3 E(Q1)
+        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s))) return rc;
+        if ((rc = stage(c, q1, q_io, q1, dt_dev, 2, s))) return rc;
+        if ((rc = stage(c, q1, q_io, q_io, dt_dev, 3, s))) return rc;
+    } else {
+        // RK2 (src/integrator.f90:62-71): Q1 = E(Q0); Q0 = 1/2 (Q0 + E(Q1))
+        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s))) return rc;
+        if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s))) return rc;
+    }
+    return HD_OK;
+}
+
+int hd_step(hd_ctx* c, double* q_io, double* q1, double* q2, double* t_dev, double* dt_dev, void* stream) {
+    if (!c || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = (int)std::min<long>(c->npart, (c->N + 255) / 256);
+    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q_io, c->grid, c->cfg.gamma, 1.01 * c->cfg.rh_floor,
+                       c->partial);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q_io, c->grid, c->cfg.gamma, c->partial, nblk,
+                       c->cfg.cflm, dt_dev, t_dev);
+    HIPCHK(hipGetLastError());
+    return hd_update(c, q_io, q1, q2, dt_dev, stream);
+}
+
+int hd_check(hd_ctx* c, void* stream, int64_t* bad_cell) {
+    if (!c) return fail(HD_ERR_INVALID, "NULL context");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    int flag = 0;
+    unsigned long long bad = ~0ull;
+    HIPCHK(hipMemcpy(&flag, c->err, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&bad, c->bad, sizeof(bad), hipMemcpyDeviceToHost));
+    if (bad_cell) *bad_cell = flag ? (int64_t)bad : -1;
+    if (flag) {
+        HIPCHK(hipMemset(c->err, 0, sizeof(int)));
+        HIPCHK(hipMemset(c->bad, 0xff, sizeof(unsigned long long)));
+        return fail(HD_ERR_NONFINITE, "NaN in mode 1 of a stage output at cell " + std::to_string(bad));
+    }
+    return HD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,474 @@
+// hd_generic.h -- table-driven HIP kernels of the DG operator, valid for every
+// supported (nbasis, iquad, nQ, Riemann solver).  One thread per cell or per
+// face; loads and stores are coalesced because consecutive threads take
+// consecutive i (the reference's fastest index).  These kernels keep the
+// reference's arithmetic order (sums over modes and points in order) and serve
+// every configuration; the fused tile kernels specialise the hot
+// configurations.
+#pragma once
+#include "hd_physics.h"
+#include "hd_tables.h"
+
+namespace hd {
+
+struct Grid {
+    int nx, ny, nz, nQ;
+    long N;                 // cells
+    double dxi, dyi, dzi;
+    int bc[6];
+};
+
+struct LimParams {
+    double rh_floor, epsP, gm1;
+};
+
+__device__ __forceinline__ long qoff(const Grid& g, long cell, int f, int r) {
+    return ((long)r * g.nQ + f) * g.N + cell;
+}
+
+// ---------------------------------------------------------------------------
+// limiter (src/flux.f90:1077-1164): one thread per cell, in place; cells the
+// limiter leaves unchanged are not written.
+// ---------------------------------------------------------------------------
+template <int NB, int QP>
+__global__ __launch_bounds__(256) void k_limiter(double* __restrict__ q, const Tables* __restrict__ T,
+                                                 Grid g, LimParams lp) {
+    constexpr int NPGE = 6 * QP * QP;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double m[5][NB];
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) m[f][r] = q[qoff(g, c, f, r)];
+    bool changed = false;
+    if (m[RH][0] < lp.rh_floor) {
+#pragma unroll
+        for (int r = 1; r < NB; r++)
+#pragma unroll
+            for (int f = 0; f < 5; f++) m[f][r] = 0.0;
+        m[RH][0] = lp.rh_floor;
+        changed = true;
+    } else {
+        const double epsi = lp.rh_floor;
+        // face points in bf_faces order: xm, xp, ym, yp, zm, zp
+        double Qrhmin = 0.;
+        for (int p = 0; p < NPGE; p++) {
+            const double* b = T->bfc[p / (QP * QP)][p % (QP * QP)];
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += b[r] * m[RH][r];
+            if (p == 0 || s < Qrhmin) Qrhmin = s;
+        }
+        if (Qrhmin < epsi) {
+            double theta = (epsi - m[RH][0]) / (Qrhmin - m[RH][0]);
+            if (theta > 1.) theta = 1.;
+            if (theta < 0.) theta = 0.;
+#pragma unroll
+            for (int r = 1; r < NB; r++) m[RH][r] = fabs(theta) * m[RH][r];
+            changed = true;
+        }
+        const double Pave =
+            lp.gm1 * (m[EN][0] - 0.5 * (m[MX][0] * m[MX][0] + m[MY][0] * m[MY][0] + m[MZ][0] * m[MZ][0]) / m[RH][0]);
+        if (Pave < lp.epsP) {
+#pragma unroll
+            for (int r = 1; r < NB; r++)
+#pragma unroll
+                for (int f = 0; f < 5; f++) m[f][r] = 0.0;
+            changed = true;
+        } else {
+            double theta = 1.;
+            for (int p = 0; p < NPGE; p++) {
+                const double* b = T->bfc[p / (QP * QP)][p % (QP * QP)];
+                double e[5];
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    double s = 0.;
+#pragma unroll
+                    for (int r = 0; r < NB; r++) s += b[r] * m[f][r];
+                    e[f] = s;
+                }
+                const double dni = 1. / e[RH];
+                const double P = lp.gm1 * (e[EN] - 0.5 * dni * (e[MX] * e[MX] + e[MY] * e[MY] + e[MZ] * e[MZ]));
+                if (P < lp.epsP && Pave != P) theta = fmin(theta, (Pave - lp.epsP) / (Pave - P));
+            }
+            if (theta > 1.) theta = 1.;
+            if (theta < 0.) theta = 0.;
+            if (theta != 1.) {
+#pragma unroll
+                for (int r = 1; r < NB; r++)
+#pragma unroll
+                    for (int f = 0; f < 5; f++) m[f][r] = theta * m[f][r];
+                changed = true;
+            }
+        }
+    }
+    if (changed) {
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) q[qoff(g, c, f, r)] = m[f][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary states (src/boundary.f90:40-338): ext from the interior trace of
+// the boundary cell.  side: 0 lo, 1 hi;  dir 0/1/2.
+// ---------------------------------------------------------------------------
+template <int NF, int NQ>
+__device__ __forceinline__ void apply_bc(int kind, int dir, int side, const double (&in)[NF][NQ],
+                                         double (&ex)[NF][NQ]) {
+    const int normal = MX + dir;
+#pragma unroll
+    for (int p = 0; p < NF; p++)
+#pragma unroll
+        for (int f = 0; f < NQ; f++) ex[p][f] = in[p][f];
+    if (kind == BC_NOSLIP) {
+#pragma unroll
+        for (int p = 0; p < NF; p++) ex[p][MX] = ex[p][MY] = ex[p][MZ] = 0.0;
+    } else if (kind == BC_WALL) {
+#pragma unroll
+        for (int p = 0; p < NF; p++) ex[p][normal] = 0.0;
+    } else if (kind == BC_OUTFLOW) {
+        if (dir == 0 && side == 1) {
+            // xhibc_outflow (src/boundary.f90:246-282) after copying the interior
+#pragma unroll
+            for (int p = 0; p < NF; p++)
+                if (ex[p][MX] > 0.0) ex[p][MX] = -in[p][MX];
+        } else if (side == 0) {
+            double v = ex[0][normal];
+#pragma unroll
+            for (int p = 1; p < NF; p++) v = fmax(v, ex[p][normal]);
+            if (v > 0.0)
+#pragma unroll
+                for (int p = 0; p < NF; p++) ex[p][normal] = 0.0;
+        } else {
+            double v = ex[0][normal];
+#pragma unroll
+            for (int p = 1; p < NF; p++) v = fmin(v, ex[p][normal]);
+            if (v < 0.0)
+#pragma unroll
+                for (int p = 0; p < NF; p++) ex[p][normal] = 0.0;
+        }
+    }
+}
+
+template <int NB, int NF, int NQ>
+__device__ __forceinline__ void load_trace(const double* __restrict__ q, const Grid& g, long cell,
+                                           const double (*bf)[10], double (&out)[NF][NQ]) {
+    double m[NB];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) m[r] = q[qoff(g, cell, f, r)];
+#pragma unroll
+        for (int p = 0; p < NF; p++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += bf[p][r] * m[r];
+            out[p][f] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// face fluxes (calc_flux_x/y/z, src/flux.f90:197-450): one thread per face of
+// direction DIR, including both domain faces; output layout
+// fl[((f*NF + p) * nfaces) + face], face = i + ni*(j + nj*k).
+// ---------------------------------------------------------------------------
+template <int NB, int QP, int NQ, int RM, int DIR>
+__global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q, double* __restrict__ fl,
+                                                   const Tables* __restrict__ T, Grid g, Phys ph) {
+    constexpr int NF = QP * QP;
+    const int ni = g.nx + (DIR == 0), nj = g.ny + (DIR == 1), nk = g.nz + (DIR == 2);
+    const long nfaces = (long)ni * nj * nk;
+    const long fid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fid >= nfaces) return;
+    const int i = (int)(fid % ni);
+    const int j = (int)((fid / ni) % nj);
+    const int k = (int)(fid / ((long)ni * nj));
+    const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int a = DIR == 0 ? i : DIR == 1 ? j : k;  // face index along the axis
+    const double(*bp)[10] = T->bfc[2 * DIR + 1];
+    const double(*bm)[10] = T->bfc[2 * DIR];
+    auto cell_at = [&](int ii, int jj, int kk) -> long { return (long)ii + (long)g.nx * ((long)jj + (long)g.ny * kk); };
+    const int il = i - (DIR == 0), jl = j - (DIR == 1), kl = k - (DIR == 2);
+    double QL[NF][NQ], QR[NF][NQ];
+    const bool periodic = g.bc[2 * DIR] == BC_PERIODIC;
+    if (a > 0) load_trace<NB, NF, NQ>(q, g, cell_at(il, jl, kl), bp, QL);
+    if (a < n_ax) load_trace<NB, NF, NQ>(q, g, cell_at(i, j, k), bm, QR);
+    if (a == 0) {
+        if (periodic) {
+            const int ii = DIR == 0 ? g.nx - 1 : i, jj = DIR == 1 ? g.ny - 1 : j, kk = DIR == 2 ? g.nz - 1 : k;
+            load_trace<NB, NF, NQ>(q, g, cell_at(ii, jj, kk), bp, QL);
+        } else {
+            apply_bc<NF, NQ>(g.bc[2 * DIR], DIR, 0, QR, QL);
+        }
+    }
+    if (a == n_ax) {
+        if (periodic) {
+            const int ii = DIR == 0 ? 0 : i, jj = DIR == 1 ? 0 : j, kk = DIR == 2 ? 0 : k;
+            load_trace<NB, NF, NQ>(q, g, cell_at(ii, jj, kk), bm, QR);
+        } else {
+            apply_bc<NF, NQ>(g.bc[2 * DIR + 1], DIR, 1, QL, QR);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NF; p++) {
+        double FL[NQ], FR[NQ], out[NQ];
+        flux_point<NQ, DIR>(ph, QL[p], FL, nullptr);
+        flux_point<NQ, DIR>(ph, QR[p], FR, nullptr);
+#pragma unroll
+        for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
+        if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL[p], QR[p], FL, FR, out);
+        if constexpr (RM == RIEMANN_ROE) roe_point<DIR>(ph, QL[p], QR[p], FL, FR, out);
+#pragma unroll
+        for (int f = 0; f < NQ; f++) fl[((long)f * NF + p) * nfaces + fid] = out[f];
+    }
+}
+
+struct StageParams {
+    const double* dt;     // device scalar
+    double coll;          // relaxation (ivis 1: implicit factor; ivis 0: explicit source)
+    int ivis;
+    int combine;          // 0: none, 1: 0.5*(qn+qs), 2: 0.75*qn+0.25*qs, 3: qn/3+2qs/3
+    int mode;             // 0: stage update, 1: write rhs = -glflux+source
+    double bf[3];         // body force
+    int* err;             // NaN flag
+    unsigned long long* bad;
+};
+
+// ---------------------------------------------------------------------------
+// cell kernel: innerintegral2 (src/flux.f90:751-928), glflux assembly
+// (src/flux.f90:958-978), source_calc (src/sources.f90:12-84) and
+// advance_time_level (src/integrator.f90:125-252), then the SSP combination of
+// RK2/RK3 (src/integrator.f90:62-86).
+// ---------------------------------------------------------------------------
+template <int NB, int QP, int NQ>
+__global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, const double* __restrict__ qn,
+                                              double* __restrict__ qout, const double* __restrict__ flx,
+                                              const double* __restrict__ fly, const double* __restrict__ flz,
+                                              const Tables* __restrict__ T, Grid g, Phys ph, StageParams sp) {
+    constexpr int NF = QP * QP, NPG = QP * QP * QP;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((long)g.nx * g.ny));
+    double m[NQ][NB];
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) m[f][r] = qin[qoff(g, c, f, r)];
+    // ---- volume integral ----
+    constexpr int NS = (NB > 4) ? 12 : 3;
+    double acc[NQ][NS];
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) acc[f][s] = 0.;
+    for (int p = 0; p < NPG; p++) {
+        const double* b = T->bint[p];
+        double Qp[NQ], Fx[NQ], Fy[NQ], Fz[NQ];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += b[r] * m[f][r];
+            Qp[f] = s;
+        }
+        flux_point<NQ, 0>(ph, Qp, Fx, nullptr);
+        flux_point<NQ, 1>(ph, Qp, Fy, nullptr);
+        flux_point<NQ, 2>(ph, Qp, Fz, nullptr);
+        const double w = T->w3[p];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            acc[f][0] += w * Fx[f];
+            acc[f][1] += w * Fy[f];
+            acc[f][2] += w * Fz[f];
+            if constexpr (NB > 4) {
+                acc[f][3] += w * b[KZ] * Fy[f];
+                acc[f][4] += w * b[KY] * Fz[f];
+                acc[f][5] += w * b[KZ] * Fx[f];
+                acc[f][6] += w * b[KX] * Fz[f];
+                acc[f][7] += w * b[KY] * Fx[f];
+                acc[f][8] += w * b[KX] * Fy[f];
+                if constexpr (NB == 10) {
+                    acc[f][9] += w * 3. * b[KX] * Fx[f];
+                    acc[f][10] += w * 3. * b[KY] * Fy[f];
+                    acc[f][11] += w * 3. * b[KZ] * Fz[f];
+                } else {
+                    acc[f][9] += w * b[KYZ] * Fx[f];
+                    acc[f][10] += w * b[KZX] * Fy[f];
+                    acc[f][11] += w * b[KXY] * Fz[f];
+                }
+            }
+        }
+    }
+    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+    // ---- face flux locations ----
+    const long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz,
+               nfz = (long)g.nx * g.ny * (g.nz + 1);
+    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
+    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
+    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+    const long fxp = fxm + 1, fyp = fym + g.nx, fzp = fzm + (long)g.nx * g.ny;
+    const double dt = *sp.dt;
+    const double faci = 1. / (1. + dt * sp.coll);
+    bool bad = false;
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        double xm[NF], xp[NF], ym[NF], yp[NF], zm[NF], zp[NF];
+#pragma unroll
+        for (int p = 0; p < NF; p++) {
+            xm[p] = flx[((long)f * NF + p) * nfx + fxm]; xp[p] = flx[((long)f * NF + p) * nfx + fxp];
+            ym[p] = fly[((long)f * NF + p) * nfy + fym]; yp[p] = fly[((long)f * NF + p) * nfy + fyp];
+            zm[p] = flz[((long)f * NF + p) * nfz + fzm]; zp[p] = flz[((long)f * NF + p) * nfz + fzp];
+        }
+        double intr[NB];
+        intr[0] = 0.;
+        intr[KX] = 0.25 * T->cb[KX] * dxi * acc[f][0];
+        intr[KY] = 0.25 * T->cb[KY] * dyi * acc[f][1];
+        intr[KZ] = 0.25 * T->cb[KZ] * dzi * acc[f][2];
+        if constexpr (NB > 4) {
+            intr[KYZ] = 0.25 * T->cb[KYZ] * (dyi * acc[f][3] + dzi * acc[f][4]);
+            intr[KZX] = 0.25 * T->cb[KZX] * (dxi * acc[f][5] + dzi * acc[f][6]);
+            intr[KXY] = 0.25 * T->cb[KXY] * (dxi * acc[f][7] + dyi * acc[f][8]);
+            if constexpr (NB == 10) {
+                intr[KXX] = 0.25 * T->cb[KXX] * (dxi * acc[f][9]);
+                intr[KYY] = 0.25 * T->cb[KYY] * (dyi * acc[f][10]);
+                intr[KZZ] = 0.25 * T->cb[KZZ] * (dzi * acc[f][11]);
+            } else {
+                intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[f][9] + dyi * acc[f][10] + dzi * acc[f][11]);
+            }
+        }
+        double glf[NB];
+        {
+            double sx = 0., sy = 0., sz = 0.;
+#pragma unroll
+            for (int p = 0; p < NF; p++) {
+                sx += T->w2[p] * (xp[p] - xm[p]);
+                sy += T->w2[p] * (yp[p] - ym[p]);
+                sz += T->w2[p] * (zp[p] - zm[p]);
+            }
+            glf[0] = 0.25 * (dxi * sx + dyi * sy + dzi * sz);
+        }
+#pragma unroll
+        for (int r = 1; r < NB; r++) {
+            double sx = 0., sy = 0., sz = 0.;
+#pragma unroll
+            for (int p = 0; p < NF; p++) {
+                sx += T->w2[p] * (T->bfc[F_XP][p][r] * xp[p] - T->bfc[F_XM][p][r] * xm[p]);
+                sy += T->w2[p] * (T->bfc[F_YP][p][r] * yp[p] - T->bfc[F_YM][p][r] * ym[p]);
+                sz += T->w2[p] * (T->bfc[F_ZP][p][r] * zp[p] - T->bfc[F_ZM][p][r] * zm[p]);
+            }
+            glf[r] = 0.25 * T->cb[r] * (dxi * sx + dyi * sy + dzi * sz) - intr[r];
+        }
+        // ---- source ----
+        double src[NB];
+#pragma unroll
+        for (int r = 0; r < NB; r++) src[r] = 0.;
+        if constexpr (NQ >= 11) {
+            if (sp.ivis == 0 && f >= PXX && sp.coll != 0.) {
+                for (int r = 0; r < NB; r++) {
+                    double s = 0.;
+                    for (int p = 0; p < NPG; p++) {
+                        double qv = 0.;
+#pragma unroll
+                        for (int rr = 0; rr < NB; rr++) qv += T->bint[p][rr] * m[f][rr];
+                        s += T->bwint[p][r] * (-sp.coll * qv);
+                    }
+                    src[r] = 0.125 * T->cb[r] * s;
+                }
+            }
+        }
+        if (f >= MX && f <= MZ && sp.bf[f - MX] != 0.)
+#pragma unroll
+            for (int r = 0; r < NB; r++) src[r] += sp.bf[f - MX] * m[RH][r];
+        if (f == EN && (sp.bf[0] != 0. || sp.bf[1] != 0. || sp.bf[2] != 0.))
+#pragma unroll
+            for (int r = 0; r < NB; r++) src[r] += sp.bf[0] * m[MX][r] + sp.bf[1] * m[MY][r] + sp.bf[2] * m[MZ][r];
+        // ---- update ----
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            const long o = qoff(g, c, f, r);
+            if (sp.mode == 1) {
+                qout[o] = -(glf[r] - src[r]);
+                continue;
+            }
+            double v = m[f][r] - dt * (glf[r] - src[r]);
+            if (sp.ivis == 1 && f >= PXX) v = faci * v;
+            if (r == 0 && v != v) bad = true;
+            if (sp.combine == 1) v = 0.5 * (qn[o] + v);
+            else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
+            else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
+            qout[o] = v;
+        }
+    }
+    if (bad) {
+        atomicOr(sp.err, 1);
+        atomicMin(sp.bad, (unsigned long long)c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// get_min_dt (src/hermeshd.f90:145-198): per-block max of the wave speed over
+// cells with rho > rh_mult*rh_floor, then a single-block finish that adds the
+// sound speed of the last cell, as the reference does.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cell_sound_speed(const double* q, const Grid& g, long c, double gamma,
+                                                   double* vmag0) {
+    const double dn = q[qoff(g, c, RH, 0)];
+    const double dni = 1. / dn;
+    const double vx = q[qoff(g, c, MX, 0)] * dni, vy = q[qoff(g, c, MY, 0)] * dni, vz = q[qoff(g, c, MZ, 0)] * dni;
+    const double cs = sqrt(gamma * (q[qoff(g, c, EN, 0)] * dni - 0.5 * (vx * vx + vy * vy + vz * vz)));
+    *vmag0 = fmax(fmax(fabs(vx) + cs, fabs(vy) + cs), fabs(vz) + cs);
+    return cs;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q, Grid g, double gamma,
+                                                   double dn_min, double* __restrict__ partial) {
+    __shared__ double red[4];
+    double vmag = 0.;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
+        double v0;
+        cell_sound_speed(q, g, c, gamma, &v0);
+        const double dn = q[qoff(g, c, RH, 0)];
+        if (v0 > vmag && dn > dn_min) vmag = v0;
+    }
+    vmag = wave_max(vmag);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmag;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = fmax(v, red[w]);
+        partial[blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ q, Grid g, double gamma,
+                                                   const double* __restrict__ partial, int npart, double cflm,
+                                                   double* __restrict__ dt_out, double* __restrict__ t_io) {
+    __shared__ double red[4];
+    double v = 0.;
+    for (int b = threadIdx.x; b < npart; b += blockDim.x) v = fmax(v, partial[b]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double vmag = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) vmag = fmax(vmag, red[w]);
+        double v0;
+        const double cs_last = cell_sound_speed(q, g, g.N - 1, gamma, &v0);
+        const double vmax = (vmag + cs_last) * g.dxi;
+        const double dt = cflm / vmax;
+        *dt_out = dt;
+        if (t_io) *t_io = *t_io + dt;
+    }
+}
+
+}  // namespace hd

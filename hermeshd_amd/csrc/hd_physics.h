@@ -1,0 +1,223 @@
+// hd_physics.h -- point-wise physics of HERMESHD's DG operator, written for
+// one quadrature point per call so that device kernels can keep everything in
+// registers.  The arithmetic follows the reference routines cited below; the
+// per-face loops of the reference are independent across points, so a
+// per-point function computes the same values.
+//
+//   flux_point  : flux_calc_pnts_r  (src/flux.f90:23-193), ivis in {0,1}, ieos=1
+//   hllc_point  : flux_hllc         (src/flux.f90:466-647), ibatten=0 (Davis speeds)
+//   roe_point   : flux_roe          (src/ces/10_moment/dg_3D_hydro_lqc_SEAN.f90:1614-1789)
+//   cfcal       : cfcal             (src/flux.f90:1029-1072)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define HD_DEV __host__ __device__ __forceinline__
+
+namespace hd {
+
+enum : int { RH = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PXX = 5, PYY = 6, PZZ = 7, PXY = 8, PXZ = 9, PYZ = 10 };
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2, BC_NOSLIP = 3 };
+enum : int { RIEMANN_CENTRAL = 0, RIEMANN_HLLC = 1, RIEMANN_ROE = 2 };
+
+struct Phys {
+    double gamma, gm1;       // aindex, aindm1
+    double P_floor, rh_floor;
+    double colvis, c2d3cv, c4d3cv;
+};
+
+// DIR: 0=x, 1=y, 2=z.  S: stochastic stress xx,yy,zz,xy,xz,yz or nullptr.
+template <int NQ, int DIR>
+HD_DEV void flux_point(const Phys& ph, const double* Q, double* F, const double* S) {
+    const double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
+    double P_xx = 0., P_yy = 0., P_zz = 0., P_xy = 0., P_xz = 0., P_yz = 0.;
+    if constexpr (NQ >= 11) {
+        P_xx = Q[PXX]; P_yy = Q[PYY]; P_zz = Q[PZZ]; P_xy = Q[PXY]; P_xz = Q[PXZ]; P_yz = Q[PYZ];
+    }
+    double Sxx = 0., Syy = 0., Szz = 0., Sxy = 0., Sxz = 0., Syz = 0.;
+    if (S) { Sxx = S[0]; Syy = S[1]; Szz = S[2]; Sxy = S[3]; Sxz = S[4]; Syz = S[5]; }
+    const double dni = 1. / dn;
+    const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
+    const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
+    double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
+    if (P < ph.P_floor) P = ph.P_floor;
+    if constexpr (DIR == 0) {
+        F[RH] = M_x;
+        F[MX] = M_x * vx + P + P_xx - Sxx;
+        F[MY] = M_y * vx + P_xy - Sxy;
+        F[MZ] = M_z * vx + P_xz - Sxz;
+        F[EN] = (Ener + P) * vx + (P_xx - Sxx) * vx + (P_xy - Sxy) * vy + (P_xz - Sxz) * vz;
+        if constexpr (NQ >= 11) {
+            F[PXX] = ph.c4d3cv * vx; F[PYY] = -ph.c2d3cv * vx; F[PZZ] = -ph.c2d3cv * vx;
+            F[PXY] = ph.colvis * vy; F[PXZ] = ph.colvis * vz; F[PYZ] = 0.;
+        }
+    } else if constexpr (DIR == 1) {
+        F[RH] = M_y;
+        F[MX] = M_x * vy + P_xy - Sxy;
+        F[MY] = M_y * vy + P + P_yy - Syy;
+        F[MZ] = M_z * vy + P_yz - Syz;
+        F[EN] = (Ener + P) * vy + (P_yy - Syy) * vy + (P_xy - Sxy) * vx + (P_yz - Syz) * vz;
+        if constexpr (NQ >= 11) {
+            F[PXX] = -ph.c2d3cv * vy; F[PYY] = ph.c4d3cv * vy; F[PZZ] = -ph.c2d3cv * vy;
+            F[PXY] = ph.colvis * vx; F[PXZ] = 0.; F[PYZ] = ph.colvis * vz;
+        }
+    } else {
+        F[RH] = M_z;
+        F[MX] = M_x * vz + P_xz - Sxz;
+        F[MY] = M_y * vz + P_yz - Syz;
+        F[MZ] = M_z * vz + P + P_zz - Szz;
+        F[EN] = (Ener + P) * vz + (P_zz - Szz) * vz + (P_xz - Sxz) * vx + (P_yz - Syz) * vy;
+        if constexpr (NQ >= 11) {
+            F[PXX] = -ph.c2d3cv * vz; F[PYY] = -ph.c2d3cv * vz; F[PZZ] = ph.c4d3cv * vz;
+            F[PXY] = 0.; F[PXZ] = ph.colvis * vx; F[PYZ] = ph.colvis * vy;
+        }
+    }
+}
+
+template <int DIR>
+HD_DEV void dir_fields(int& ipar, int& ip1, int& ip2) {
+    if constexpr (DIR == 0) { ipar = MX; ip1 = MY; ip2 = MZ; }
+    else if constexpr (DIR == 1) { ipar = MY; ip1 = MZ; ip2 = MX; }
+    else { ipar = MZ; ip1 = MX; ip2 = MY; }
+}
+
+// HLLC at one face point.  QL/QR: conserved traces (>= 5 fields), FL/FR their
+// physical fluxes (same direction).  Writes out[0..4] in field order
+// rh,mx,my,mz,en.  On entry out[] holds the fallback (central) flux, which is
+// kept when no branch of Eq. (26) applies (only possible with NaN speeds).
+template <int DIR>
+HD_DEV void hllc_point(const Phys& ph, const double* QL, const double* QR, const double* FL,
+                       const double* FR, double* out) {
+    int ipar, ip1, ip2;
+    dir_fields<DIR>(ipar, ip1, ip2);
+    const double gm1 = ph.gm1, g = ph.gamma;
+    // primitive states (src/flux.f90:510-520)
+    const double rL = QL[RH], rR = QR[RH];
+    const double riL = 1.0 / rL, riR = 1.0 / rR;
+    const double uL = QL[ipar] * riL, vL = QL[ip1] * riL, wL = QL[ip2] * riL;
+    const double uR = QR[ipar] * riR, vR = QR[ip1] * riR, wR = QR[ip2] * riR;
+    const double pL = gm1 * (QL[EN] - 0.5 * rL * (uL * uL + vL * vL + wL * wL));
+    const double pR = gm1 * (QR[EN] - 0.5 * rR * (uR * uR + vR * vR + wR * wR));
+    // Davis wave speeds (src/flux.f90:522-585)
+    const double cL = sqrt(g * pL / rL), cR = sqrt(g * pR / rR);
+    const double SL = fmin(uL - cL, uR - cR);
+    const double SR = fmax(uR + cR, uL + cL);
+    // contact speed and star pressure, Eqs. (34),(36) of Batten 1997
+    double sm_num = rR * uR * (SR - uR) - rL * uL * (SL - uL);
+    sm_num = sm_num + pL - pR;
+    double sm_den = rR * (SR - uR) - rL * (SL - uL);
+    if (sm_den == 0.0) sm_den = ph.rh_floor;
+    const double SM = sm_num / sm_den;
+    const double pstar = rL * (uL - SL) * (uL - SM) + pL;
+    // Eq. (26) in the reference's sequential-if order: the last true branch wins
+    int sel = -1;
+    if (SL > 0.0) sel = 0;
+    if (SL <= 0.0 && 0.0 < SM) sel = 1;
+    if (SM <= 0.0 && 0.0 <= SR) sel = 2;
+    if (SR < 0.0) sel = 3;
+    if (sel == 0) {
+        out[RH] = FL[RH]; out[MX] = FL[MX]; out[MY] = FL[MY]; out[MZ] = FL[MZ]; out[EN] = FL[EN];
+    } else if (sel == 3) {
+        out[RH] = FR[RH]; out[MX] = FR[MX]; out[MY] = FR[MY]; out[MZ] = FR[MZ]; out[EN] = FR[EN];
+    } else if (sel > 0) {
+        // star state of the selected side, Eqs. (35)-(40), and F* = F + S (U* - U), Eq. (29)
+        const bool left = (sel == 1);
+        const double* Qk = left ? QL : QR;
+        const double* Fk = left ? FL : FR;
+        const double Sk = left ? SL : SR;
+        const double uk = left ? uL : uR, rk = left ? rL : rR, pk = left ? pL : pR;
+        double den = Sk - SM;
+        if (den == 0.0) den = ph.rh_floor;
+        const double slrm_i = 1.0 / den;
+        const double sq = Sk - uk;
+        const double s0 = rk * sq * slrm_i;
+        const double s1 = (sq * Qk[ipar] + pstar - pk) * slrm_i;
+        const double s2 = sq * Qk[ip1] * slrm_i;
+        const double s3 = sq * Qk[ip2] * slrm_i;
+        const double s4 = (sq * Qk[EN] - pk * uk + pstar * SM) * slrm_i;
+        out[RH] = Fk[RH] + Sk * (s0 - Qk[RH]);
+        out[ipar] = Fk[ipar] + Sk * (s1 - Qk[ipar]);
+        out[ip1] = Fk[ip1] + Sk * (s2 - Qk[ip1]);
+        out[ip2] = Fk[ip2] + Sk * (s3 - Qk[ip2]);
+        out[EN] = Fk[EN] + Sk * (s4 - Qk[EN]);
+    }
+}
+
+// freezing speed |u_dir| + c (cfcal, ivis in {0,1}, ieos = 1)
+template <int DIR>
+HD_DEV double cfcal(const Phys& ph, const double* Q) {
+    const double dn = Q[RH], dni = 1. / dn;
+    const double vx = Q[MX] * dni, vy = Q[MY] * dni, vz = Q[MZ] * dni;
+    const double P = ph.gm1 * (Q[EN] - 0.5 * dn * (vx * vx + vy * vy + vz * vz));
+    const double cs = sqrt(ph.gamma * P * dni);
+    const double v = DIR == 0 ? vx : DIR == 1 ? vy : vz;
+    return fabs(v) + cs;
+}
+
+// Roe flux (Eulderink & Mellema) for rh..en at one point; when the Roe sound
+// speed squared is not positive, local Lax-Friedrichs with cfcal speeds, as the
+// legacy flux_cal does (dg_3D_hydro_lqc_SEAN.f90:1255-1289).
+template <int DIR>
+HD_DEV void roe_point(const Phys& ph, const double* QL, const double* QR, const double* FL,
+                      const double* FR, double* out) {
+    int ipar, ip1, ip2;
+    dir_fields<DIR>(ipar, ip1, ip2);
+    const double gm1 = ph.gm1, cp = ph.gamma / gm1;
+    double sk[2], sw[2][4];
+    for (int s = 0; s < 2; s++) {
+        const double* Q = s ? QR : QL;
+        const double dnii = 1. / Q[RH];
+        const double a = Q[ipar] * dnii, b = Q[ip1] * dnii, c = Q[ip2] * dnii;
+        const double vsq = a * a + b * b + c * c;
+        const double Pri = gm1 * (Q[EN] - 0.5 * Q[RH] * vsq);
+        sk[s] = sqrt(Q[RH]);
+        sw[s][0] = sk[s] * a; sw[s][1] = sk[s] * b; sw[s][2] = sk[s] * c;
+        sw[s][3] = 0.5 * sk[s] * (vsq + cp * Pri / Q[RH]);
+    }
+    const double Qri = 1. / QL[RH];
+    const double delrho = QR[RH] * Qri - 1.;
+    const double dele = (QR[EN] - QL[EN]) * Qri;
+    const double dm1 = (QR[ipar] - QL[ipar]) * Qri;
+    const double dm2 = (QR[ip1] - QL[ip1]) * Qri;
+    const double dm3 = (QR[ip2] - QL[ip2]) * Qri;
+    const double dsk = 0.5 * (sk[1] + sk[0]);
+    const double vc0 = 0.5 * (sw[1][0] + sw[0][0]) / dsk;
+    const double vc1 = 0.5 * (sw[1][1] + sw[0][1]) / dsk;
+    const double vc2 = 0.5 * (sw[1][2] + sw[0][2]) / dsk;
+    const double hc = 0.5 * (sw[1][3] + sw[0][3]) / dsk;
+    const double vcsq = vc0 * vc0 + vc1 * vc1 + vc2 * vc2;
+    const double asq = gm1 * (hc - 0.5 * vcsq);
+    if (!(asq > 0.0)) {
+        const double cw = fmax(cfcal<DIR>(ph, QL), cfcal<DIR>(ph, QR));
+        const int idx[5] = {RH, MX, MY, MZ, EN};
+        for (int e = 0; e < 5; e++) {
+            const int f = idx[e];
+            out[f] = 0.5 * (FL[f] + FR[f]) - 0.5 * cw * (QR[f] - QL[f]);
+        }
+        return;
+    }
+    const double sc = sqrt(asq);
+    const double l0 = fabs(vc0 - sc), l1 = fabs(vc0 + sc), l2 = fabs(vc0);
+    const double ea2i = gm1 / asq;
+    const double vcdel = vc0 * dm1 + vc1 * dm2 + vc2 * dm3 - dele;
+    const double a9e = 0.5 * ea2i * (0.5 * vcsq * delrho - vcdel);
+    const double sk9 = 0.5 * (dm1 - vc0 * delrho) / sc;
+    const double a0 = a9e - sk9, a1 = a9e + sk9;
+    const double a2 = ea2i * ((hc - vcsq) * delrho + vcdel);
+    const double a3 = dm2 - vc1 * delrho, a4 = dm3 - vc2 * delrho;
+    // Delta F = sum_j a_j |lambda_j| e_j with e_j columns over rows
+    // (rho, energy, parallel, perp1, perp2)
+    const double t0 = a0 * l0, t1 = a1 * l1, t2 = a2 * l2, t3 = a3 * l2, t4 = a4 * l2;
+    double d_rho = t0 + t1 + t2;
+    double d_en = t0 * (hc - sc * vc0) + t1 * (hc + sc * vc0) + t2 * (0.5 * vcsq) + t3 * vc1 + t4 * vc2;
+    double d_par = t0 * (vc0 - sc) + t1 * (vc0 + sc) + t2 * vc0;
+    double d_p1 = t0 * vc1 + t1 * vc1 + t2 * vc1 + t3;
+    double d_p2 = t0 * vc2 + t1 * vc2 + t2 * vc2 + t4;
+    const double rl = QL[RH];
+    out[RH] = 0.5 * (FL[RH] + FR[RH] - d_rho * rl);
+    out[EN] = 0.5 * (FL[EN] + FR[EN] - d_en * rl);
+    out[ipar] = 0.5 * (FL[ipar] + FR[ipar] - d_par * rl);
+    out[ip1] = 0.5 * (FL[ip1] + FR[ip1] - d_p1 * rl);
+    out[ip2] = 0.5 * (FL[ip2] + FR[ip2] - d_p2 * rl);
+}
+
+}  // namespace hd

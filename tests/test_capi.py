@@ -1,0 +1,67 @@
+"""The C-ABI library loads and exports every symbol include/hermeshd_amd.h
+declares (CPU: no compute calls), and the host-side config logic."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from hermeshd_amd import _lib
+from hermeshd_amd.config import HDConfig, sod_1d, vortex_3d
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hd_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_symbols()
+    assert set(_lib.EXPORTED) <= names, set(_lib.EXPORTED) - names
+    assert names <= set(_lib.EXPORTED) | {n for n in names if n.startswith("hd_ens_") or n.startswith("hd_adj")}
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_config_struct_layout(tmp_path):
+    """The ctypes mirror of hd_config matches the C layout (offsetof from gcc)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    fields = [f[0] for f in _lib.HDConfigC._fields_]
+    src = tmp_path / "probe.c"
+    body = "\n".join(f'printf("{f} %zu\\n", offsetof(hd_config, {f}));' for f in fields)
+    src.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "hermeshd_amd.h"\n'
+                   f'int main(void){{ {body} printf("sizeof %zu\\n", sizeof(hd_config)); return 0; }}\n')
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = dict(line.split() for line in subprocess.check_output([str(exe)]).decode().splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(_lib.HDConfigC, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(_lib.HDConfigC)
+
+
+def test_config_validation():
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, nbasis=10, iquad=2)
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, bc=("periodic", "wall", "periodic", "periodic", "periodic", "periodic"))
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, ivis=1, nQ=5)
+    c = vortex_3d()
+    assert c.shape == (10, 5, 32, 128, 128) and c.cflm == 0.08
+    assert abs(1.0 / c.dxi - 10.0 / 128) < 1e-15
+    assert sod_1d().cflm == 0.3
