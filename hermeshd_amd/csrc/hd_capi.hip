@@ -14,7 +14,8 @@
 #include <vector>
 
 #include "../../include/hermeshd_amd.h"
-#include "hd_generic.h"
+#include "hd_fused.h"
+#include <cstdlib>
 
 namespace {
 
@@ -49,6 +50,8 @@ struct hd_ctx {
     int* err = nullptr;
     unsigned long long* bad = nullptr;
     double* scratch_dt = nullptr;
+    double* stage_buf = nullptr;  // third stage array of RK3 when the caller passes none
+    bool generic = false;         // HD_KERNELS=generic selects the table-driven kernels
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage
@@ -69,6 +72,7 @@ struct Ops {
     void (*limiter)(hd_ctx*, double*, hipStream_t);
     void (*faces)(hd_ctx*, const double*, hipStream_t);
     void (*cell)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
+    void (*fused)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
 };
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -76,7 +80,10 @@ inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs
 template <int NB, int QP, int NQ, int RM>
 struct Impl {
     static void limiter(hd_ctx* c, double* q, hipStream_t s) {
-        hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
+        if (c->generic)
+            hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
+        else
+            hipLaunchKernelGGL((fz::k_limit<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->grid, c->lp);
     }
     static void faces(hd_ctx* c, const double* q, hipStream_t s) {
         const Grid& g = c->grid;
@@ -89,7 +96,11 @@ struct Impl {
         hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
                            c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
     }
-    static Ops ops() { return Ops{&limiter, &faces, &cell}; }
+    static void fused(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, qin, qn, qout,
+                           c->grid, c->ph, sp);
+    }
+    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused}; }
 };
 
 template <int NB, int QP, int NQ>
@@ -156,6 +167,8 @@ StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int
     for (int d = 0; d < 3; d++) sp.bf[d] = c->cfg.body_force[d];
     sp.err = c->err;
     sp.bad = c->bad;
+    sp.limit_out = 0;
+    sp.lp = c->lp;
     return sp;
 }
 
@@ -168,8 +181,11 @@ hipEvent_t next_event(hd_ctx* c) {
     return c->ev_pool[c->ev_used++];
 }
 
-// one Euler stage with optional fused SSP combination
-int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* dt_dev, int combine, hipStream_t s) {
+// One Euler stage with optional fused SSP combination.  in_limited: the input
+// was already limited by the previous stage's epilogue.  limit_out: the output
+// is the next stage's input, so the fused kernel limits it before storing.
+int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* dt_dev, int combine, hipStream_t s,
+          bool in_limited = false, bool limit_out = false) {
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -178,10 +194,16 @@ int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* 
         if (!e0 || !e1) return fail(HD_ERR_HIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, s));
     }
-    if (c->cfg.limiter) o.limiter(c, qin, s);
-    o.faces(c, qin, s);
+    const bool fuse_lim = c->cfg.limiter && !c->generic;
+    if (c->cfg.limiter && !(in_limited && fuse_lim)) o.limiter(c, qin, s);
     StageParams sp = stage_params(c, dt_dev, combine, 0);
-    o.cell(c, qin, qn, qout, sp, s);
+    sp.limit_out = (limit_out && fuse_lim) ? 1 : 0;
+    if (c->generic) {
+        o.faces(c, qin, s);
+        o.cell(c, qin, qn, qout, sp, s);
+    } else {
+        o.fused(c, qin, qn, qout, sp, s);
+    }
     HIPCHK(hipGetLastError());
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, s));
@@ -232,7 +254,9 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         (r = alloc(c, (void**)&c->partial, c->npart * sizeof(double))) ||
         (r = alloc(c, (void**)&c->err, sizeof(int))) ||
         (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
-        (r = alloc(c, (void**)&c->scratch_dt, sizeof(double)))) {
+        (r = alloc(c, (void**)&c->scratch_dt, sizeof(double))) ||
+        (cfg->integrator == 3 &&
+         (r = alloc(c, (void**)&c->stage_buf, (size_t)c->N * c->nQ * c->nb * sizeof(double))))) {
         std::string msg = g_last_error;
         hd_destroy(c);
         return fail(r, msg);
@@ -243,6 +267,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         hd_destroy(c);
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
+    const char* kern = std::getenv("HD_KERNELS");
+    c->generic = kern && std::strcmp(kern, "generic") == 0;
     *out = c;
     return HD_OK;
 }
@@ -250,7 +276,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
 int hd_destroy(hd_ctx* c) {
     if (!c) return HD_OK;
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    void* ptrs[] = {c->dtab, c->fl[0], c->fl[1], c->fl[2], c->partial, c->err, c->bad, c->scratch_dt};
+    void* ptrs[] = {c->dtab, c->fl[0], c->fl[1], c->fl[2], c->partial, c->err, c->bad, c->scratch_dt, c->stage_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -295,10 +321,15 @@ int hd_rhs(hd_ctx* c, const double* q, double* rhs, void* stream) {
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     hipStream_t s = (hipStream_t)stream;
-    o.faces(c, q, s);
+    if (q == rhs) return fail(HD_ERR_INVALID, "q and rhs must differ");
     // dt is not used in rhs mode; point at the context's scratch scalar
     StageParams sp = stage_params(c, c->scratch_dt, 0, 1);
-    o.cell(c, q, q, rhs, sp, s);
+    if (c->generic) {
+        o.faces(c, q, s);
+        o.cell(c, q, q, rhs, sp, s);
+    } else {
+        o.fused(c, q, q, rhs, sp, s);
+    }
     HIPCHK(hipGetLastError());
     return HD_OK;
 }
@@ -322,20 +353,21 @@ int hd_min_dt(hd_ctx* c, const double* q, double* dt_dev, void* stream) {
 }
 
 int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_dev, void* stream) {
-    (void)q2;
     if (!c || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
-    if (q_io == q1) return fail(HD_ERR_INVALID, "q_io and q1 must differ");
+    if (q_io == q1 || (q2 && (q2 == q_io || q2 == q1))) return fail(HD_ERR_INVALID, "q_io, q1 and q2 must differ");
     hipStream_t s = (hipStream_t)stream;
     int rc;
     if (c->cfg.integrator == 3) {
-        // RK3 (src/integrator.f90:74-86): Q1 = E(Q0); Q1 = 3/4 Q0 + 1/4 E(Q1); Q0 = 1/3 Q0 + 2/3 E(Q1)
-        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s))) return rc;
-        if ((rc = stage(c, q1, q_io, q1, dt_dev, 2, s))) return rc;
-        if ((rc = stage(c, q1, q_io, q_io, dt_dev, 3, s))) return rc;
+        // RK3 (src/integrator.f90:74-86): Q1 = E(Q0); Q2 = 3/4 Q0 + 1/4 E(Q1); Q0 = 1/3 Q0 + 2/3 E(Q2).
+        // A stage never writes the array it reads neighbours from.
+        double* q2b = q2 ? q2 : c->stage_buf;
+        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if ((rc = stage(c, q1, q_io, q2b, dt_dev, 2, s, true, true))) return rc;
+        if ((rc = stage(c, q2b, q_io, q_io, dt_dev, 3, s, true, false))) return rc;
     } else {
         // RK2 (src/integrator.f90:62-71): Q1 = E(Q0); Q0 = 1/2 (Q0 + E(Q1))
-        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s))) return rc;
-        if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s))) return rc;
+        if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
     }
     return HD_OK;
 }

@@ -1,0 +1,474 @@
+// hd_fused.h -- one fused HIP kernel per SSP-RK stage: volume integral, face
+// traces, Riemann fluxes on all six faces, surface lift, sources, update and
+// the SSP combination, for one cell per thread with the state in registers.
+//
+// Operator (the reference's glflux + innerintegral2 + source_calc +
+// advance_time_level, src/flux.f90:751-1022, src/sources.f90:12-84,
+// src/integrator.f90:125-252), with dQ/dt = -glflux + source:
+//   RHS[f][r] = sum_d dxi_d * ( sum_g Cvol(d,r,g) F_d(Q(g))[f]
+//                             + sum_p Cface(-d,r,p) Fhat_{-d}(p)[f]
+//                             + sum_p Cface(+d,r,p) Fhat_{+d}(p)[f] )  + source
+//   Cvol  = 0.25 cb_r w3_g dphi_r/dxi_d(g),  Cface(-d) = +0.25 cb_r w2_p phi_r(-d,p),
+//   Cface(+d) = -0.25 cb_r w2_p phi_r(+d,p).
+// The basis is the Legendre product of src/basis_funcs.f90 with the mode order
+// of src/initialize.f90:126-194; quadrature nodes and weights are the
+// reference's values (1/sqrt(3); sqrt(15)/5, 5/9, 8/9) as exact hex literals.
+// Each interior face's flux is evaluated by both adjacent cells (same inputs,
+// same arithmetic, so both sides use the same value).
+#pragma once
+#include "hd_generic.h"
+#include "hd_prim.h"
+
+namespace hd {
+namespace fz {
+
+// ---------------------------------------------------------------- rules ----
+template <int QP>
+struct Rule;
+template <>
+struct Rule<2> {
+    static HD_DEV constexpr double x(int i) { return i == 0 ? -0x1.279a74590331dp-1 : 0x1.279a74590331dp-1; }
+    static HD_DEV constexpr double w(int) { return 1.0; }
+    static HD_DEV constexpr double p2(int) { return 0x1.0p-53; }  // 1.5*s*s - 0.5 at s = 1/sqrt(3)
+    static HD_DEV constexpr double w2(int, int) { return 1.0; }
+    static HD_DEV constexpr double w3(int, int, int) { return 1.0; }
+};
+template <>
+struct Rule<3> {
+    static HD_DEV constexpr double x(int i) {
+        return i == 0 ? -0x1.8c97ef43f7248p-1 : (i == 1 ? 0.0 : 0x1.8c97ef43f7248p-1);
+    }
+    static HD_DEV constexpr double w(int i) { return i == 1 ? 0x1.c71c71c71c71cp-1 : 0x1.1c71c71c71c72p-1; }
+    static HD_DEV constexpr double p2(int i) { return i == 1 ? -0.5 : 0x1.999999999999cp-2; }
+    // set_weights_3D: w2(i,j) = w1(j)*w1(i); w3(i; j,k) = w2(j,k)*w1(i)
+    static HD_DEV constexpr double w2(int i, int j) { return w(j) * w(i); }
+    static HD_DEV constexpr double w3(int i, int j, int k) { return w2(j, k) * w(i); }
+};
+
+// ---------------------------------------------------------------- basis ----
+// Legendre degrees (a,b,c) in (x,y,z) of mode r
+template <int NB>
+HD_DEV constexpr int deg(int r, int ax) {
+    // 1, x, y, z, yz, zx, xy, then xyz (NB 8) or P2x, P2y, P2z (NB 10); pure
+    // arithmetic so that no lookup array is ever materialised on the device
+    return r == 0 ? 0
+         : r <= 3 ? (ax == r - 1 ? 1 : 0)
+         : r <= 6 ? (ax == r - 4 ? 0 : 1)
+         : (NB == 8 ? 1 : (ax == r - 7 ? 2 : 0));
+}
+template <int NB>
+HD_DEV constexpr double cbasis(int r) {
+    return double((2 * deg<NB>(r, 0) + 1) * (2 * deg<NB>(r, 1) + 1) * (2 * deg<NB>(r, 2) + 1));
+}
+template <int QP>
+HD_DEV constexpr double Lval(int a, int i) { return a == 0 ? 1.0 : (a == 1 ? Rule<QP>::x(i) : Rule<QP>::p2(i)); }
+// value at the face where the coordinate is -1 (side 0) or +1 (side 1)
+HD_DEV constexpr double Lface(int a, int side) { return a == 1 ? (side ? 1.0 : -1.0) : 1.0; }
+template <int QP>
+HD_DEV constexpr double Dval(int a, int i) { return a == 0 ? 0.0 : (a == 1 ? 1.0 : 3.0 * Rule<QP>::x(i)); }
+
+// interior point g = (i,j,k), x index slowest
+template <int NB, int QP>
+HD_DEV constexpr double phi_int(int r, int i, int j, int k) {
+    return Lval<QP>(deg<NB>(r, 0), i) * Lval<QP>(deg<NB>(r, 1), j) * Lval<QP>(deg<NB>(r, 2), k);
+}
+// 0.25*cb_r*w3_g*d(phi_r)/d(xi_d) at g
+template <int NB, int QP>
+HD_DEV constexpr double cvol(int d, int r, int i, int j, int k) {
+    const double a = d == 0 ? Dval<QP>(deg<NB>(r, 0), i) : Lval<QP>(deg<NB>(r, 0), i);
+    const double b = d == 1 ? Dval<QP>(deg<NB>(r, 1), j) : Lval<QP>(deg<NB>(r, 1), j);
+    const double c = d == 2 ? Dval<QP>(deg<NB>(r, 2), k) : Lval<QP>(deg<NB>(r, 2), k);
+    return 0.25 * cbasis<NB>(r) * Rule<QP>::w3(i, j, k) * (a * b * c);
+}
+// face point p = (ii,jj), jj fastest: x faces (y=x[jj], z=x[ii]); y faces
+// (x=x[ii], z=x[jj]); z faces (x=x[jj], y=x[ii])  (src/basis_funcs.f90:346-567)
+template <int NB, int QP>
+HD_DEV constexpr double phi_face(int dir, int side, int r, int ii, int jj) {
+    const int a = deg<NB>(r, 0), b = deg<NB>(r, 1), c = deg<NB>(r, 2);
+    if (dir == 0) return Lface(a, side) * Lval<QP>(b, jj) * Lval<QP>(c, ii);
+    if (dir == 1) return Lval<QP>(a, ii) * Lface(b, side) * Lval<QP>(c, jj);
+    return Lval<QP>(a, jj) * Lval<QP>(b, ii) * Lface(c, side);
+}
+template <int NB, int QP>
+HD_DEV constexpr double cface(int dir, int side, int r, int ii, int jj) {
+    return (side ? -0.25 : 0.25) * cbasis<NB>(r) * Rule<QP>::w2(ii, jj) * phi_face<NB, QP>(dir, side, r, ii, jj);
+}
+
+// Face tables in constant memory for the runtime point loops:
+//   bf[dir][side][p][r] = phi_r on that face at point p
+//   cf[dir][side][p][r] = cface
+template <int NB, int QP>
+struct FaceTab {
+    double bf[3][2][QP * QP][NB];
+    double cf[3][2][QP * QP][NB];
+};
+template <int NB, int QP>
+constexpr FaceTab<NB, QP> make_face_tab() {
+    FaceTab<NB, QP> t{};
+    for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++)
+            for (int ii = 0; ii < QP; ii++)
+                for (int jj = 0; jj < QP; jj++)
+                    for (int r = 0; r < NB; r++) {
+                        t.bf[d][s][ii * QP + jj][r] = phi_face<NB, QP>(d, s, r, ii, jj);
+                        t.cf[d][s][ii * QP + jj][r] = cface<NB, QP>(d, s, r, ii, jj);
+                    }
+    return t;
+}
+__constant__ FaceTab<4, 2> c_face_4 = make_face_tab<4, 2>();
+__constant__ FaceTab<8, 2> c_face_8 = make_face_tab<8, 2>();
+__constant__ FaceTab<10, 3> c_face_10 = make_face_tab<10, 3>();
+template <int NB>
+__device__ __forceinline__ const FaceTab<NB, (NB == 10 ? 3 : 2)>& face_tab();
+template <>
+__device__ __forceinline__ const FaceTab<4, 2>& face_tab<4>() { return c_face_4; }
+template <>
+__device__ __forceinline__ const FaceTab<8, 2>& face_tab<8>() { return c_face_8; }
+template <>
+__device__ __forceinline__ const FaceTab<10, 3>& face_tab<10>() { return c_face_10; }
+
+// Keeps the scheduler from interleaving fully unrolled quadrature points,
+// which would stretch live ranges past the register file.
+__device__ __forceinline__ void point_barrier() { __builtin_amdgcn_sched_barrier(0); }
+
+// ------------------------------------------------------------ volume -------
+// innerintegral2 (src/flux.f90:751-928): Q at every Gauss point, the three
+// directional fluxes from one set of primitives, and the projection onto
+// d(phi_r)/d(xi_d) with compile-time weights (zero weights drop out).
+template <int NB, int QP, int NQ>
+__device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
+                                            double dxi, double dyi, double dzi) {
+#pragma unroll
+    for (int i = 0; i < QP; i++)
+#pragma unroll
+        for (int j = 0; j < QP; j++)
+#pragma unroll
+            for (int k = 0; k < QP; k++) {
+                double Qp[NQ];
+#pragma unroll
+                for (int f = 0; f < NQ; f++) {
+                    double s = 0.;
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        const double b = phi_int<NB, QP>(r, i, j, k);
+                        if (b != 0.0) s += b * m[f][r];
+                    }
+                    Qp[f] = s;
+                }
+                const pr::Prim st = pr::prim<NQ>(ph, Qp);
+                double Fx[NQ], Fy[NQ], Fz[NQ];
+                pr::flux<NQ, 0>(ph, Qp, st, Fx, nullptr);
+                pr::flux<NQ, 1>(ph, Qp, st, Fy, nullptr);
+                pr::flux<NQ, 2>(ph, Qp, st, Fz, nullptr);
+#pragma unroll
+                for (int f = 0; f < NQ; f++) {
+                    const double fx = dxi * Fx[f], fy = dyi * Fy[f], fz = dzi * Fz[f];
+#pragma unroll
+                    for (int r = 1; r < NB; r++) {
+                        const double cx = cvol<NB, QP>(0, r, i, j, k);
+                        const double cy = cvol<NB, QP>(1, r, i, j, k);
+                        const double cz = cvol<NB, QP>(2, r, i, j, k);
+                        if (cx != 0.0) rhs[f][r] += cx * fx;
+                        if (cy != 0.0) rhs[f][r] += cy * fy;
+                        if (cz != 0.0) rhs[f][r] += cz * fz;
+                    }
+                }
+                point_barrier();
+            }
+}
+
+// ------------------------------------------------------------- faces -------
+struct CellPos {
+    int i, j, k;
+    long c;
+};
+
+template <int NB, int NQ>
+__device__ __forceinline__ void load_modes(const double* __restrict__ q, const Grid& g, long c, double (&m)[NQ][NB]) {
+#pragma unroll
+    for (int r = 0; r < NB; r++)
+#pragma unroll
+        for (int f = 0; f < NQ; f++) m[f][r] = q[((long)r * NQ + f) * g.N + c];
+}
+
+// One face of the cell: DIR 0/1/2, SIDE 0 (minus) / 1 (plus).
+template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
+__device__ __forceinline__ void face_term(const double* __restrict__ q, const Grid& g, const CellPos& cp,
+                                          const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
+                                          double dxi_d) {
+    constexpr int NF = QP * QP;
+    const auto& T = face_tab<NB>();
+    const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int a = DIR == 0 ? cp.i : DIR == 1 ? cp.j : cp.k;
+    const bool at_edge = SIDE ? (a == n_ax - 1) : (a == 0);
+    const int bc = g.bc[2 * DIR + SIDE];
+    const bool ext = at_edge && bc != BC_PERIODIC;
+    double nm[NQ][NB];
+    if (!ext) {
+        int an = SIDE ? a + 1 : a - 1;
+        if (an < 0) an += n_ax;
+        if (an >= n_ax) an -= n_ax;
+        const long cn = DIR == 0 ? (cp.c - cp.i + an)
+                      : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
+                                 : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
+        load_modes<NB, NQ>(q, g, cn, nm);
+    }
+    // outflow boundaries decide on the whole face (src/boundary.f90:213-338)
+    bool zero_normal = false;
+    if (ext && bc == BC_OUTFLOW && !(DIR == 0 && SIDE == 1)) {
+        double v = 0.;
+        for (int p = 0; p < NF; p++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += T.bf[DIR][SIDE][p][r] * m[MX + DIR][r];
+            v = (p == 0) ? s : (SIDE ? fmin(v, s) : fmax(v, s));
+        }
+        zero_normal = SIDE ? (v < 0.0) : (v > 0.0);
+    }
+#pragma unroll 1
+    for (int p = 0; p < NF; p++) {
+        double own[NQ], oth[NQ];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += T.bf[DIR][SIDE][p][r] * m[f][r];
+            own[f] = s;
+        }
+        if (!ext) {
+#pragma unroll
+            for (int f = 0; f < NQ; f++) {
+                double s = 0.;
+#pragma unroll
+                for (int r = 0; r < NB; r++) s += T.bf[DIR][1 - SIDE][p][r] * nm[f][r];
+                oth[f] = s;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < NQ; f++) oth[f] = own[f];
+            if (bc == BC_NOSLIP) {
+                oth[MX] = oth[MY] = oth[MZ] = 0.0;
+            } else if (bc == BC_WALL) {
+                oth[MX + DIR] = 0.0;
+            } else if (bc == BC_OUTFLOW) {
+                if (DIR == 0 && SIDE == 1) {
+                    if (oth[MX] > 0.0) oth[MX] = -own[MX];
+                } else if (zero_normal) {
+                    oth[MX + DIR] = 0.0;
+                }
+            }
+        }
+        const double* QL = SIDE ? own : oth;
+        const double* QR = SIDE ? oth : own;
+        double FL[NQ], FR[NQ], out[NQ];
+        if constexpr (RM == RIEMANN_ROE) {
+            flux_point<NQ, DIR>(ph, QL, FL, nullptr);
+            flux_point<NQ, DIR>(ph, QR, FR, nullptr);
+#pragma unroll
+            for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
+            roe_point<DIR>(ph, QL, QR, FL, FR, out);
+        } else {
+            const pr::Prim sL = pr::prim<NQ>(ph, QL), sR = pr::prim<NQ>(ph, QR);
+            pr::flux<NQ, DIR>(ph, QL, sL, FL, nullptr);
+            pr::flux<NQ, DIR>(ph, QR, sR, FR, nullptr);
+#pragma unroll
+            for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
+            if constexpr (RM == RIEMANN_HLLC) pr::hllc<DIR>(ph, QL, sL, QR, sR, FL, FR, out);
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            const double v = dxi_d * out[f];
+#pragma unroll
+            for (int r = 0; r < NB; r++) rhs[f][r] += T.cf[DIR][SIDE][p][r] * v;
+        }
+    }
+}
+
+// ----------------------------------------------------------- limiter -------
+// limiter (src/flux.f90:1077-1164), in place, compile-time face tables.  The
+// pressure step takes the minimum face pressure first and divides once:
+// (Pave-epsP)/(Pave-P) is monotone in P for P < epsP <= Pave, so this equals
+// the reference's minimum over points.
+// Limits the modes m[0..4][*] of one cell in place; returns true if changed.
+template <int NB, int QP, int MQ>
+__device__ __forceinline__ bool limit_cell(double (&m)[MQ][NB], const LimParams& lp) {
+    bool changed = false;
+    if (m[RH][0] < lp.rh_floor) {
+#pragma unroll
+        for (int r = 1; r < NB; r++)
+#pragma unroll
+            for (int f = 0; f < 5; f++) m[f][r] = 0.0;
+        m[RH][0] = lp.rh_floor;
+        changed = true;
+    } else {
+        const auto& T = face_tab<NB>();
+        double rmin = 0.;
+#pragma unroll 1
+        for (int p = 0; p < 6 * QP * QP; p++) {
+            const double* b = T.bf[p / (2 * QP * QP)][(p / (QP * QP)) % 2][p % (QP * QP)];
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += b[r] * m[RH][r];
+            rmin = (p == 0) ? s : (s < rmin ? s : rmin);
+        }
+        if (rmin < lp.rh_floor) {
+            double theta = (lp.rh_floor - m[RH][0]) / (rmin - m[RH][0]);
+            if (theta > 1.) theta = 1.;
+            if (theta < 0.) theta = 0.;
+#pragma unroll
+            for (int r = 1; r < NB; r++) m[RH][r] = fabs(theta) * m[RH][r];
+            changed = true;
+        }
+        const double Pave =
+            lp.gm1 * (m[EN][0] - 0.5 * (m[MX][0] * m[MX][0] + m[MY][0] * m[MY][0] + m[MZ][0] * m[MZ][0]) / m[RH][0]);
+        if (Pave < lp.epsP) {
+#pragma unroll
+            for (int r = 1; r < NB; r++)
+#pragma unroll
+                for (int f = 0; f < 5; f++) m[f][r] = 0.0;
+            changed = true;
+        } else {
+            double pmin = 0.;
+#pragma unroll 1
+            for (int p = 0; p < 6 * QP * QP; p++) {
+                const double* b = T.bf[p / (2 * QP * QP)][(p / (QP * QP)) % 2][p % (QP * QP)];
+                double e[5];
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    double s = 0.;
+#pragma unroll
+                    for (int r = 0; r < NB; r++) s += b[r] * m[f][r];
+                    e[f] = s;
+                }
+                const double dni = pr::rcp(e[RH]);
+                const double P = lp.gm1 * (e[EN] - 0.5 * dni * (e[MX] * e[MX] + e[MY] * e[MY] + e[MZ] * e[MZ]));
+                pmin = (p == 0) ? P : (P < pmin ? P : pmin);
+            }
+            if (pmin < lp.epsP && Pave != pmin) {
+                double theta = (Pave - lp.epsP) / (Pave - pmin);
+                if (theta > 1.) theta = 1.;
+                if (theta < 0.) theta = 0.;
+                if (theta != 1.) {
+#pragma unroll
+                    for (int r = 1; r < NB; r++)
+#pragma unroll
+                        for (int f = 0; f < 5; f++) m[f][r] = theta * m[f][r];
+                    changed = true;
+                }
+            }
+        }
+    }
+    return changed;
+}
+
+template <int NB, int QP>
+__global__ __launch_bounds__(256, (NB == 4 ? 4 : 2)) void k_limit(double* __restrict__ q, Grid g, LimParams lp) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double m[5][NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++)
+#pragma unroll
+        for (int f = 0; f < 5; f++) m[f][r] = q[((long)r * g.nQ + f) * g.N + c];
+    if (limit_cell<NB, QP, 5>(m, lp)) {
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+#pragma unroll
+            for (int f = 0; f < 5; f++) q[((long)r * g.nQ + f) * g.N + c] = m[f][r];
+    }
+}
+
+// ------------------------------------------------------------ kernel -------
+__device__ __forceinline__ void fence() { asm volatile("" ::: "memory"); }
+
+template <int NB, int QP, int NQ, int RM>
+__global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
+                                               double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    CellPos cp;
+    cp.c = c;
+    cp.i = (int)(c % g.nx);
+    cp.j = (int)((c / g.nx) % g.ny);
+    cp.k = (int)(c / ((long)g.nx * g.ny));
+    double m[NQ][NB];
+    load_modes<NB, NQ>(qin, g, c, m);
+    double rhs[NQ][NB];
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) rhs[f][r] = 0.;
+    // The fences keep each section's loads inside it, so the compiler does not
+    // hoist all six neighbours' modes at once (which would spill).
+    volume_term<NB, QP, NQ>(m, rhs, ph, g.dxi, g.dyi, g.dzi);
+    fence();
+    face_term<NB, QP, NQ, RM, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi);
+    fence();
+    face_term<NB, QP, NQ, RM, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi);
+    fence();
+    face_term<NB, QP, NQ, RM, 1, 0>(qin, g, cp, m, rhs, ph, g.dyi);
+    fence();
+    face_term<NB, QP, NQ, RM, 1, 1>(qin, g, cp, m, rhs, ph, g.dyi);
+    fence();
+    face_term<NB, QP, NQ, RM, 2, 0>(qin, g, cp, m, rhs, ph, g.dzi);
+    fence();
+    face_term<NB, QP, NQ, RM, 2, 1>(qin, g, cp, m, rhs, ph, g.dzi);
+    fence();
+    // sources (src/sources.f90:12-84): ivis 0 relaxes the stress explicitly
+    // (-coll*P, projected exactly in modal space); body force rho*g, m.g
+    if constexpr (NQ >= 11) {
+        if (sp.ivis == 0 && sp.coll != 0.)
+#pragma unroll
+            for (int f = PXX; f <= PYZ; f++)
+#pragma unroll
+                for (int r = 0; r < NB; r++) rhs[f][r] += -sp.coll * m[f][r];
+    }
+    if (sp.bf[0] != 0. || sp.bf[1] != 0. || sp.bf[2] != 0.) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            rhs[MX][r] += sp.bf[0] * m[RH][r];
+            rhs[MY][r] += sp.bf[1] * m[RH][r];
+            rhs[MZ][r] += sp.bf[2] * m[RH][r];
+            rhs[EN][r] += sp.bf[0] * m[MX][r] + sp.bf[1] * m[MY][r] + sp.bf[2] * m[MZ][r];
+        }
+    }
+    // advance_time_level + SSP combination
+    const double dt = *sp.dt;
+    const double faci = 1. / (1. + dt * sp.coll);
+    bool bad = false;
+    if (sp.mode == 1) {
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) qout[((long)r * NQ + f) * g.N + c] = rhs[f][r];
+        return;
+    }
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            const long o = ((long)r * NQ + f) * g.N + c;
+            double v = m[f][r] + dt * rhs[f][r];
+            if (sp.ivis == 1 && f >= PXX) v = faci * v;
+            if (r == 0 && v != v) bad = true;
+            if (sp.combine == 1) v = 0.5 * (qn[o] + v);
+            else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
+            else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
+            rhs[f][r] = v;  // reuse the accumulator as the output cell
+        }
+    // When this output is the next stage's input, apply that stage's limiter
+    // here: the reference limits the stage input in place before anything
+    // reads it (src/integrator.f90:106-113), so the stored values are the same.
+    if (sp.limit_out) limit_cell<NB, QP, NQ>(rhs, sp.lp);
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) qout[((long)r * NQ + f) * g.N + c] = rhs[f][r];
+    if (bad) {
+        atomicOr(sp.err, 1);
+        atomicMin(sp.bad, (unsigned long long)c);
+    }
+}
+
+}  // namespace fz
+}  // namespace hd

@@ -236,6 +236,8 @@ struct StageParams {
     double bf[3];         // body force
     int* err;             // NaN flag
     unsigned long long* bad;
+    int limit_out;        // fused kernels: limit the output (it is the next stage's input)
+    LimParams lp;
 };
 
 // ---------------------------------------------------------------------------

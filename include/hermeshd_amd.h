@@ -88,10 +88,11 @@ int hd_euler_step(hd_ctx *ctx, double *q_in, double *q_out, const double *dt_dev
 int hd_min_dt(hd_ctx *ctx, const double *q, double *dt_dev, void *stream);
 
 /* One SSP-RK2 or SSP-RK3 step with a given dt.  Replaces RK2/RK3
-   (src/integrator.f90:62-86).  q1 is used as stage storage; q2 may be NULL
-   (the stage combinations are fused into the stage kernels, so the third
-   array of the reference is not needed).  On return q_io holds the new state;
-   q1 and q2 hold unspecified values. */
+   (src/integrator.f90:62-86).  q1 and q2 are stage storage as in the
+   reference; q2 may be NULL (SSP-RK2 does not need it, and for SSP-RK3 the
+   context's own stage array is used).  The SSP combinations are fused into
+   the stage kernels.  On return q_io holds the new state; q1 and q2 hold
+   unspecified values. */
 int hd_update(hd_ctx *ctx, double *q_io, double *q1, double *q2, const double *dt_dev, void *stream);
 
 /* dt = min_dt(q_io); update; t += dt.  t_dev and dt_dev are device doubles.
