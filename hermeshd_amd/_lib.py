@@ -17,7 +17,8 @@ HD_ERR_UNSUPPORTED = -4
 # every extern "C" symbol declared in include/hermeshd_amd.h
 EXPORTED = [
     "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_update",
-    "hd_step", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes", "hd_last_error",
+    "hd_step", "hd_set_step", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
+    "hd_last_error",
 ]
 
 
@@ -66,6 +67,7 @@ def lib() -> C.CDLL:
     L.hd_min_dt.argtypes = [vp, vp, vp, vp]
     L.hd_update.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hd_step.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.hd_set_step.argtypes = [vp, C.c_uint64, vp]
     L.hd_check.argtypes = [vp, vp, C.POINTER(C.c_int64)]
     L.hd_timing_enable.argtypes = [vp, C.c_int]
     L.hd_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int64)]

@@ -15,21 +15,40 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhermeshd_amd.so")
 ARCH = os.environ.get("HD_OFFLOAD_ARCH", "gfx950")
 
-HIPCC_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+COMPILE_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # keep packed-FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+LINK_FLAGS = ["-shared", "-fPIC", f"--offload-arch={ARCH}"]
 
 
 def sources():
     return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, jobs: int | None = None) -> str:
+    """Compiles each .hip unit to an object in parallel, then links the library."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp", *sources()]
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    procs, objs = [], []
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    for src in sources():
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        objs.append(obj)
+        cmd = [hipcc, *COMPILE_FLAGS, *inc, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        while len([p for p in procs if p.poll() is None]) >= jobs:
+            procs[[p.poll() is None for p in procs].index(True)].wait()
+        procs.append(subprocess.Popen(cmd))
+    failed = [p.args[-1] for p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed for " + ", ".join(failed))
+    cmd = [hipcc, *LINK_FLAGS, "-o", LIB + ".tmp", *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

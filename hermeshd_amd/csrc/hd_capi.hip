@@ -1,9 +1,6 @@
 // hd_capi.hip -- C ABI of the HERMESHD hot path on MI355X (gfx950).
 //
-// A context owns the basis tables, geometry and scratch of one configuration
-// (the reference keeps these as Fortran module globals; src/params.f90,
-// src/basis_funcs.f90, src/boundary.f90, src/flux.f90:11-18).  All entry
-// points are stream-ordered and return an error code; none aborts.
+// All entry points are stream-ordered and return an error code; none aborts.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -13,9 +10,11 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/hermeshd_amd.h"
-#include "hd_fused.h"
+#include <cmath>
 #include <cstdlib>
+
+#include "hd_ctx.h"
+#include "hd_noise.h"
 
 namespace {
 
@@ -35,95 +34,17 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
-struct hd_ctx {
-    hd_config cfg;
-    int q, nb, nQ, nface;
-    long N;
-    hd::Grid grid;
-    hd::Phys ph;
-    hd::LimParams lp;
-    hd::Tables htab;
-    hd::Tables* dtab = nullptr;
-    double* fl[3] = {nullptr, nullptr, nullptr};  // face fluxes per direction
-    double* partial = nullptr;                    // wave-speed partial maxima
-    int npart = 0;
-    int* err = nullptr;
-    unsigned long long* bad = nullptr;
-    double* scratch_dt = nullptr;
-    double* stage_buf = nullptr;  // third stage array of RK3 when the caller passes none
-    bool generic = false;         // HD_KERNELS=generic selects the table-driven kernels
-    int64_t bytes = 0;
-    int device = 0;
-    // stage timing (hd_timing_enable): event pairs recorded around each stage
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
-    size_t ev_used = 0;
-};
 
 namespace {
 
 using namespace hd;
 
-// ---------------------------------------------------------------------------
-// template dispatch over (nbasis, iquad) x nQ x Riemann solver
-// ---------------------------------------------------------------------------
-struct Ops {
-    void (*limiter)(hd_ctx*, double*, hipStream_t);
-    void (*faces)(hd_ctx*, const double*, hipStream_t);
-    void (*cell)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
-    void (*fused)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
-};
-
-inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
-template <int NB, int QP, int NQ, int RM>
-struct Impl {
-    static void limiter(hd_ctx* c, double* q, hipStream_t s) {
-        if (c->generic)
-            hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
-        else
-            hipLaunchKernelGGL((fz::k_limit<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->grid, c->lp);
-    }
-    static void faces(hd_ctx* c, const double* q, hipStream_t s) {
-        const Grid& g = c->grid;
-        long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz, nfz = (long)g.nx * g.ny * (g.nz + 1);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph);
-    }
-    static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
-        hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
-                           c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
-    }
-    static void fused(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
-        hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, qin, qn, qout,
-                           c->grid, c->ph, sp);
-    }
-    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused}; }
-};
-
-template <int NB, int QP, int NQ>
-bool pick_rm(int rm, Ops* o) {
-    switch (rm) {
-        case RIEMANN_CENTRAL: *o = Impl<NB, QP, NQ, RIEMANN_CENTRAL>::ops(); return true;
-        case RIEMANN_HLLC: *o = Impl<NB, QP, NQ, RIEMANN_HLLC>::ops(); return true;
-        case RIEMANN_ROE: *o = Impl<NB, QP, NQ, RIEMANN_ROE>::ops(); return true;
-    }
-    return false;
-}
-
-template <int NB, int QP>
-bool pick_nq(int nq, int rm, Ops* o) {
-    if (nq == 5) return pick_rm<NB, QP, 5>(rm, o);
-    if (nq == 11) return pick_rm<NB, QP, 11>(rm, o);
-    return false;
-}
-
+// dispatch over (nbasis, iquad) x nQ x Riemann solver
 bool pick_ops(const hd_ctx* c, Ops* o) {
-    if (c->nb == 4 && c->q == 2) return pick_nq<4, 2>(c->nQ, c->cfg.riemann, o);
-    if (c->nb == 8 && c->q == 2) return pick_nq<8, 2>(c->nQ, c->cfg.riemann, o);
-    if (c->nb == 10 && c->q == 3) return pick_nq<10, 3>(c->nQ, c->cfg.riemann, o);
+    if (c->nb == 4 && c->q == 2) return hd_pick_4(c->nQ, c->cfg.riemann, o);
+    if (c->nb == 8 && c->q == 2) return hd_pick_8(c->nQ, c->cfg.riemann, o);
+    if (c->nb == 10 && c->q == 3 && c->nQ == 5) return hd_pick_10_5(c->cfg.riemann, o);
+    if (c->nb == 10 && c->q == 3 && c->nQ == 11) return hd_pick_10_11(c->cfg.riemann, o);
     return false;
 }
 
@@ -145,7 +66,7 @@ int validate(const hd_config* c) {
         if ((c->bc[2 * d] == HD_BC_PERIODIC) != (c->bc[2 * d + 1] == HD_BC_PERIODIC))
             return fail(HD_ERR_INVALID, "a periodic axis must be periodic on both sides");
     }
-    if (c->noise) return fail(HD_ERR_UNSUPPORTED, "stochastic stress is served by the ensemble entry points");
+    if (c->noise && c->nQ != 11) return fail(HD_ERR_INVALID, "noise needs nQ=11 (the stress fields)");
     if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) return fail(HD_ERR_INVALID, "domain lengths must be positive");
     if (!(c->gamma > 1.0)) return fail(HD_ERR_INVALID, "gamma must exceed 1");
     return HD_OK;
@@ -169,7 +90,52 @@ StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int
     sp.bad = c->bad;
     sp.limit_out = 0;
     sp.lp = c->lp;
+    for (int d = 0; d < 3; d++) sp.noise[d] = c->noise_f[d];
+    sp.noise[3] = c->noise_v;
+    sp.nfb = c->nfb;
     return sp;
+}
+
+// Draws the stochastic stress of the current step for time step *dt_dev.
+int draw_noise(hd_ctx* c, const double* dt_dev, hipStream_t s) {
+    if (!c->cfg.noise) return HD_OK;
+    noise::NoiseParams np;
+    np.seed = c->cfg.seed;
+    np.realization = c->cfg.realization;
+    np.nface = c->nface;
+    np.npg = c->nface * c->q;
+    np.step = c->step_ctr;
+    np.dt = dt_dev;
+    np.eta_sd = c->cfg.eta_sd;
+    np.zeta_sd = c->cfg.zeta_sd;
+    np.dVi = c->grid.dxi * c->grid.dyi * c->grid.dzi;
+    np.kap_f = c->kap_f;
+    np.kap_v = c->kap_v;
+    np.nfb = c->nfb;
+    np.N = c->N;
+    for (int d = 0; d < 3; d++) np.face[d] = c->noise_f[d];
+    np.vol = c->noise_v;
+    const long total = 3L * c->nface * c->nfb + (long)np.npg * c->N;
+    hipLaunchKernelGGL(noise::k_noise, dim3(blocks_for(total, 256)), dim3(256), 0, s, np);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int advance_step(hd_ctx* c, hipStream_t s) {
+    if (!c->cfg.noise) return HD_OK;
+    hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, c->step_ctr, 0ull, 1);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+// kappa = 1/sqrt(sum_p (w_p/W)^2) of a product rule in 2 (face) or 3 (volume) dimensions
+double rule_kappa(const Tables& t, int dims) {
+    const int n = dims == 2 ? t.nface : t.npg;
+    const double* w = dims == 2 ? t.w2 : t.w3;
+    double W = 0., s2 = 0.;
+    for (int p = 0; p < n; p++) W += w[p];
+    for (int p = 0; p < n; p++) s2 += (w[p] / W) * (w[p] / W);
+    return 1.0 / std::sqrt(s2);
 }
 
 hipEvent_t next_event(hd_ctx* c) {
@@ -255,13 +221,32 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         (r = alloc(c, (void**)&c->err, sizeof(int))) ||
         (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
         (r = alloc(c, (void**)&c->scratch_dt, sizeof(double))) ||
+        (r = alloc(c, (void**)&c->step_ctr, sizeof(unsigned long long))) ||
         (cfg->integrator == 3 &&
          (r = alloc(c, (void**)&c->stage_buf, (size_t)c->N * c->nQ * c->nb * sizeof(double))))) {
         std::string msg = g_last_error;
         hd_destroy(c);
         return fail(r, msg);
     }
+    if (cfg->noise) {
+        // the step's stress field: 6 components at every point of the three
+        // face lattices and of the volume rule
+        c->nfb = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        c->kap_f = rule_kappa(c->htab, 2);
+        c->kap_v = rule_kappa(c->htab, 3);
+        const size_t nfw = (size_t)6 * c->nface * c->nfb, nvw = (size_t)6 * c->htab.npg * c->N;
+        for (int d = 0; d < 3 && r == HD_OK; d++) r = alloc(c, (void**)&c->noise_f[d], nfw * sizeof(double));
+        if (r == HD_OK) r = alloc(c, (void**)&c->noise_v, nvw * sizeof(double));
+        if (r) {
+            std::string msg = g_last_error;
+            hd_destroy(c);
+            return fail(r, msg);
+        }
+    }
+    const double one = 1.0;
     if (hipMemcpy(c->dtab, &c->htab, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->scratch_dt, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->step_ctr, 0, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(c->err, 0, sizeof(int)) != hipSuccess ||
         hipMemset(c->bad, 0xff, sizeof(unsigned long long)) != hipSuccess) {
         hd_destroy(c);
@@ -269,6 +254,10 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     }
     const char* kern = std::getenv("HD_KERNELS");
     c->generic = kern && std::strcmp(kern, "generic") == 0;
+    if (c->generic && cfg->noise) {
+        hd_destroy(c);
+        return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
+    }
     *out = c;
     return HD_OK;
 }
@@ -276,7 +265,9 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
 int hd_destroy(hd_ctx* c) {
     if (!c) return HD_OK;
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    void* ptrs[] = {c->dtab, c->fl[0], c->fl[1], c->fl[2], c->partial, c->err, c->bad, c->scratch_dt, c->stage_buf};
+    void* ptrs[] = {c->dtab,     c->fl[0],    c->fl[1],     c->fl[2],      c->partial,    c->err,
+                    c->bad,      c->scratch_dt, c->stage_buf, c->step_ctr, c->noise_f[0], c->noise_f[1],
+                    c->noise_f[2], c->noise_v};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -322,7 +313,10 @@ int hd_rhs(hd_ctx* c, const double* q, double* rhs, void* stream) {
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     hipStream_t s = (hipStream_t)stream;
     if (q == rhs) return fail(HD_ERR_INVALID, "q and rhs must differ");
-    // dt is not used in rhs mode; point at the context's scratch scalar
+    // rhs mode does not advance; the scratch scalar (1.0) is the time step of
+    // the noise amplitude
+    int rc = draw_noise(c, c->scratch_dt, s);
+    if (rc) return rc;
     StageParams sp = stage_params(c, c->scratch_dt, 0, 1);
     if (c->generic) {
         o.faces(c, q, s);
@@ -337,6 +331,8 @@ int hd_rhs(hd_ctx* c, const double* q, double* rhs, void* stream) {
 int hd_euler_step(hd_ctx* c, double* q_in, double* q_out, const double* dt_dev, void* stream) {
     if (!c || !q_in || !q_out || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
     if (q_in == q_out) return fail(HD_ERR_INVALID, "q_in and q_out must differ");
+    int rc = draw_noise(c, dt_dev, (hipStream_t)stream);
+    if (rc) return rc;
     return stage(c, q_in, q_in, q_out, dt_dev, 0, (hipStream_t)stream);
 }
 
@@ -357,6 +353,8 @@ int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_
     if (q_io == q1 || (q2 && (q2 == q_io || q2 == q1))) return fail(HD_ERR_INVALID, "q_io, q1 and q2 must differ");
     hipStream_t s = (hipStream_t)stream;
     int rc;
+    // one stress field per step, shared by its stages
+    if ((rc = draw_noise(c, dt_dev, s))) return rc;
     if (c->cfg.integrator == 3) {
         // RK3 (src/integrator.f90:74-86): Q1 = E(Q0); Q2 = 3/4 Q0 + 1/4 E(Q1); Q0 = 1/3 Q0 + 2/3 E(Q2).
         // A stage never writes the array it reads neighbours from.
@@ -369,6 +367,14 @@ int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_
         if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
         if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
     }
+    return advance_step(c, s);
+}
+
+int hd_set_step(hd_ctx* c, uint64_t step, void* stream) {
+    if (!c) return fail(HD_ERR_INVALID, "NULL context");
+    hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, (hipStream_t)stream, c->step_ctr,
+                       (unsigned long long)step, 0);
+    HIPCHK(hipGetLastError());
     return HD_OK;
 }
 

@@ -115,9 +115,9 @@ constexpr FaceTab<NB, QP> make_face_tab() {
                     }
     return t;
 }
-__constant__ FaceTab<4, 2> c_face_4 = make_face_tab<4, 2>();
-__constant__ FaceTab<8, 2> c_face_8 = make_face_tab<8, 2>();
-__constant__ FaceTab<10, 3> c_face_10 = make_face_tab<10, 3>();
+static __constant__ FaceTab<4, 2> c_face_4 = make_face_tab<4, 2>();
+static __constant__ FaceTab<8, 2> c_face_8 = make_face_tab<8, 2>();
+static __constant__ FaceTab<10, 3> c_face_10 = make_face_tab<10, 3>();
 template <int NB>
 __device__ __forceinline__ const FaceTab<NB, (NB == 10 ? 3 : 2)>& face_tab();
 template <>
@@ -135,9 +135,10 @@ __device__ __forceinline__ void point_barrier() { __builtin_amdgcn_sched_barrier
 // innerintegral2 (src/flux.f90:751-928): Q at every Gauss point, the three
 // directional fluxes from one set of primitives, and the projection onto
 // d(phi_r)/d(xi_d) with compile-time weights (zero weights drop out).
-template <int NB, int QP, int NQ>
+template <int NB, int QP, int NQ, bool NZ>
 __device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
-                                            double dxi, double dyi, double dzi) {
+                                            double dxi, double dyi, double dzi, const StageParams& sp, long c,
+                                            long N) {
 #pragma unroll
     for (int i = 0; i < QP; i++)
 #pragma unroll
@@ -157,9 +158,19 @@ __device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&
                 }
                 const pr::Prim st = pr::prim<NQ>(ph, Qp);
                 double Fx[NQ], Fy[NQ], Fz[NQ];
-                pr::flux<NQ, 0>(ph, Qp, st, Fx, nullptr);
-                pr::flux<NQ, 1>(ph, Qp, st, Fy, nullptr);
-                pr::flux<NQ, 2>(ph, Qp, st, Fz, nullptr);
+                double S[6];
+                const double* Sp = nullptr;
+                if constexpr (NZ) {
+                    // volume point (i,j,k) of the cell: stream id p*N + c
+                    const int p = (i * QP + j) * QP + k;
+                    constexpr long npg = QP * QP * QP;
+#pragma unroll
+                    for (int e = 0; e < 6; e++) S[e] = sp.noise[3][((long)e * npg + p) * N + c];
+                    Sp = S;
+                }
+                pr::flux<NQ, 0>(ph, Qp, st, Fx, Sp);
+                pr::flux<NQ, 1>(ph, Qp, st, Fy, Sp);
+                pr::flux<NQ, 2>(ph, Qp, st, Fz, Sp);
 #pragma unroll
                 for (int f = 0; f < NQ; f++) {
                     const double fx = dxi * Fx[f], fy = dyi * Fy[f], fz = dzi * Fz[f];
@@ -192,10 +203,10 @@ __device__ __forceinline__ void load_modes(const double* __restrict__ q, const G
 }
 
 // One face of the cell: DIR 0/1/2, SIDE 0 (minus) / 1 (plus).
-template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
+template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE>
 __device__ __forceinline__ void face_term(const double* __restrict__ q, const Grid& g, const CellPos& cp,
                                           const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
-                                          double dxi_d) {
+                                          double dxi_d, const StageParams& sp) {
     constexpr int NF = QP * QP;
     const auto& T = face_tab<NB>();
     const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
@@ -224,6 +235,14 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
             v = (p == 0) ? s : (SIDE ? fmin(v, s) : fmax(v, s));
         }
         zero_normal = SIDE ? (v < 0.0) : (v > 0.0);
+    }
+    long fid = 0;
+    if constexpr (NZ) {
+        // lattice id of this face; a periodic high face is the low face
+        int af = a + SIDE;
+        if (af == n_ax && bc == BC_PERIODIC) af = 0;
+        const int fi = DIR == 0 ? af : cp.i, fj = DIR == 1 ? af : cp.j, fk = DIR == 2 ? af : cp.k;
+        fid = (long)fi + (long)(g.nx + 1) * ((long)fj + (long)(g.ny + 1) * fk);
     }
 #pragma unroll 1
     for (int p = 0; p < NF; p++) {
@@ -260,17 +279,24 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
         }
         const double* QL = SIDE ? own : oth;
         const double* QR = SIDE ? oth : own;
+        double S[6];
+        const double* Sp = nullptr;
+        if constexpr (NZ) {
+#pragma unroll
+            for (int e = 0; e < 6; e++) S[e] = sp.noise[DIR][((long)e * NF + p) * sp.nfb + fid];
+            Sp = S;
+        }
         double FL[NQ], FR[NQ], out[NQ];
         if constexpr (RM == RIEMANN_ROE) {
-            flux_point<NQ, DIR>(ph, QL, FL, nullptr);
-            flux_point<NQ, DIR>(ph, QR, FR, nullptr);
+            flux_point<NQ, DIR>(ph, QL, FL, Sp);
+            flux_point<NQ, DIR>(ph, QR, FR, Sp);
 #pragma unroll
             for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
             roe_point<DIR>(ph, QL, QR, FL, FR, out);
         } else {
             const pr::Prim sL = pr::prim<NQ>(ph, QL), sR = pr::prim<NQ>(ph, QR);
-            pr::flux<NQ, DIR>(ph, QL, sL, FL, nullptr);
-            pr::flux<NQ, DIR>(ph, QR, sR, FR, nullptr);
+            pr::flux<NQ, DIR>(ph, QL, sL, FL, Sp);
+            pr::flux<NQ, DIR>(ph, QR, sR, FR, Sp);
 #pragma unroll
             for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
             if constexpr (RM == RIEMANN_HLLC) pr::hllc<DIR>(ph, QL, sL, QR, sR, FL, FR, out);
@@ -381,7 +407,7 @@ __global__ __launch_bounds__(256, (NB == 4 ? 4 : 2)) void k_limit(double* __rest
 // ------------------------------------------------------------ kernel -------
 __device__ __forceinline__ void fence() { asm volatile("" ::: "memory"); }
 
-template <int NB, int QP, int NQ, int RM>
+template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
                                                double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -400,19 +426,19 @@ __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, c
         for (int r = 0; r < NB; r++) rhs[f][r] = 0.;
     // The fences keep each section's loads inside it, so the compiler does not
     // hoist all six neighbours' modes at once (which would spill).
-    volume_term<NB, QP, NQ>(m, rhs, ph, g.dxi, g.dyi, g.dzi);
+    volume_term<NB, QP, NQ, NZ>(m, rhs, ph, g.dxi, g.dyi, g.dzi, sp, c, g.N);
     fence();
-    face_term<NB, QP, NQ, RM, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi);
+    face_term<NB, QP, NQ, RM, NZ, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi, sp);
     fence();
-    face_term<NB, QP, NQ, RM, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi);
+    face_term<NB, QP, NQ, RM, NZ, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi, sp);
     fence();
-    face_term<NB, QP, NQ, RM, 1, 0>(qin, g, cp, m, rhs, ph, g.dyi);
+    face_term<NB, QP, NQ, RM, NZ, 1, 0>(qin, g, cp, m, rhs, ph, g.dyi, sp);
     fence();
-    face_term<NB, QP, NQ, RM, 1, 1>(qin, g, cp, m, rhs, ph, g.dyi);
+    face_term<NB, QP, NQ, RM, NZ, 1, 1>(qin, g, cp, m, rhs, ph, g.dyi, sp);
     fence();
-    face_term<NB, QP, NQ, RM, 2, 0>(qin, g, cp, m, rhs, ph, g.dzi);
+    face_term<NB, QP, NQ, RM, NZ, 2, 0>(qin, g, cp, m, rhs, ph, g.dzi, sp);
     fence();
-    face_term<NB, QP, NQ, RM, 2, 1>(qin, g, cp, m, rhs, ph, g.dzi);
+    face_term<NB, QP, NQ, RM, NZ, 2, 1>(qin, g, cp, m, rhs, ph, g.dzi, sp);
     fence();
     // sources (src/sources.f90:12-84): ivis 0 relaxes the stress explicitly
     // (-coll*P, projected exactly in modal space); body force rho*g, m.g

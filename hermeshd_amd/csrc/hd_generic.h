@@ -238,6 +238,8 @@ struct StageParams {
     unsigned long long* bad;
     int limit_out;        // fused kernels: limit the output (it is the next stage's input)
     LimParams lp;
+    const double* noise[4];  // stochastic stress of the step: x, y, z face families, volume points
+    long nfb;                // face lattice size (nx+1)(ny+1)(nz+1)
 };
 
 // ---------------------------------------------------------------------------
@@ -432,7 +434,7 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q, Grid g, double gamma,
+static __global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q, Grid g, double gamma,
                                                    double dn_min, double* __restrict__ partial) {
     __shared__ double red[4];
     double vmag = 0.;
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q,
     }
 }
 
-__global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ q, Grid g, double gamma,
+static __global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ q, Grid g, double gamma,
                                                    const double* __restrict__ partial, int npart, double cflm,
                                                    double* __restrict__ dt_out, double* __restrict__ t_io) {
     __shared__ double red[4];

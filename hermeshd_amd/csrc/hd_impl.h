@@ -1,0 +1,52 @@
+// hd_impl.h -- launch wrappers of one (nbasis, iquad, nQ, Riemann solver)
+// configuration; included by the instantiation units only.
+#pragma once
+#include "hd_ctx.h"
+#include "hd_fused.h"
+
+namespace hd {
+
+template <int NB, int QP, int NQ, int RM>
+struct Impl {
+    static void limiter(hd_ctx* c, double* q, hipStream_t s) {
+        if (c->generic)
+            hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
+        else
+            hipLaunchKernelGGL((fz::k_limit<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->grid, c->lp);
+    }
+    static void faces(hd_ctx* c, const double* q, hipStream_t s) {
+        const Grid& g = c->grid;
+        long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz, nfz = (long)g.nx * g.ny * (g.nz + 1);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph);
+    }
+    static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
+                           c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
+    }
+    static void fused(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        if constexpr (NQ >= 11) {
+            if (sp.noise[0]) {
+                hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, true>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s,
+                                   qin, qn, qout, c->grid, c->ph, sp);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, false>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, qin, qn,
+                           qout, c->grid, c->ph, sp);
+    }
+    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused}; }
+};
+
+template <int NB, int QP, int NQ>
+bool pick_rm(int rm, Ops* o) {
+    switch (rm) {
+        case RIEMANN_CENTRAL: *o = Impl<NB, QP, NQ, RIEMANN_CENTRAL>::ops(); return true;
+        case RIEMANN_HLLC: *o = Impl<NB, QP, NQ, RIEMANN_HLLC>::ops(); return true;
+        case RIEMANN_ROE: *o = Impl<NB, QP, NQ, RIEMANN_ROE>::ops(); return true;
+    }
+    return false;
+}
+
+}  // namespace hd

@@ -104,6 +104,10 @@ class Solver:
         _lib.check(_lib.lib().hd_step(self._h, self._ptr(q_io), self._ptr(q1, "q1"), p2,
                                       C.c_void_p(t.data_ptr()), C.c_void_p(dt.data_ptr()), _stream()))
 
+    def set_step(self, step: int) -> None:
+        """Step index that keys the stochastic stress (advanced by every update)."""
+        _lib.check(_lib.lib().hd_set_step(self._h, int(step), _stream()))
+
     def check(self) -> None:
         """Synchronise and raise NonFiniteError if a stage produced NaN."""
         bad = C.c_int64(-1)

@@ -49,7 +49,7 @@ typedef struct hd_config {
     int32_t limiter;             /* 1 = positivity limiter on every stage input */
     int32_t integrator;          /* 2 = heun (SSP-RK2), 3 = shu-osher (SSP-RK3) */
     int32_t bc[6];               /* xlo,xhi,ylo,yhi,zlo,zhi: HD_BC_* */
-    int32_t noise;               /* 1 = stochastic stress (needs nQ = 11) */
+    int32_t noise;               /* 1 = stochastic stress in the fluxes (needs nQ = 11) */
     int32_t reserved0[2];        /* zero; keeps the doubles 8-byte aligned without padding */
     double lx, ly, lz;           /* domain lengths, grid centred on the origin */
     double gamma;                /* aindex */
@@ -74,7 +74,9 @@ int hd_limit(hd_ctx *ctx, double *q, void *stream);
 
 /* rhs = -glflux + source for an (already limited) state.  Replaces calc_rhs
    (src/integrator.f90:116-122): glflux (src/flux.f90:932-1022) with
-   exchange_flux/apply_boundaries, and source_calc (src/sources.f90:12-84). */
+   exchange_flux/apply_boundaries, and source_calc (src/sources.f90:12-84).
+   With noise on, the stochastic stress of the current step index enters the
+   fluxes with the amplitude of a time step of 1. */
 int hd_rhs(hd_ctx *ctx, const double *q, double *rhs, void *stream);
 
 /* One forward-Euler stage: limiter on q_in (in place), then
@@ -92,13 +94,20 @@ int hd_min_dt(hd_ctx *ctx, const double *q, double *dt_dev, void *stream);
    reference; q2 may be NULL (SSP-RK2 does not need it, and for SSP-RK3 the
    context's own stage array is used).  The SSP combinations are fused into
    the stage kernels.  On return q_io holds the new state; q1 and q2 hold
-   unspecified values. */
+   unspecified values.  With noise on, one stochastic stress field is drawn
+   for the step (key seed, realization, step index; amplitude from *dt_dev),
+   used by every stage, and the step index then advances by one. */
 int hd_update(hd_ctx *ctx, double *q_io, double *q1, double *q2, const double *dt_dev, void *stream);
 
 /* dt = min_dt(q_io); update; t += dt.  t_dev and dt_dev are device doubles.
    Replaces step (src/hermeshd.f90:53-62). */
 int hd_step(hd_ctx *ctx, double *q_io, double *q1, double *q2, double *t_dev, double *dt_dev,
             void *stream);
+
+/* Sets the step index that keys the stochastic stress (0 after hd_create),
+   so that a restarted run draws the same noise.  Replaces the stream set-up
+   of random_init (src/random.f90:36-39). */
+int hd_set_step(hd_ctx *ctx, uint64_t step, void *stream);
 
 /* Waits for the context's work on `stream` and reports the NaN flag raised by
    any stage since the last check (HD_ERR_NONFINITE, with the first bad cell

@@ -223,6 +223,7 @@ void orc_destroy(orc_ctx *c) {
 }
 
 int orc_set_step_index(orc_ctx *c, uint64_t step) { c->step_index = step; return 0; }
+int orc_set_noise_dt(orc_ctx *c, double dt) { c->noise_dt = dt; return 0; }
 
 int orc_tables(const orc_ctx *c, double *xq, double *w1, double *w2, double *w3, double *cbasis,
                double *bint, double *bxp, double *bxm, double *byp, double *bym, double *bzp,
@@ -693,27 +694,36 @@ static void exchange_and_bc(orc_ctx *c, const double *q) {
 /* ------------------------------------------------------------------------ */
 /* stochastic stress at points (src/random.f90:106-145 intent, FDT-scaled)   */
 /* ------------------------------------------------------------------------ */
-/* Symmetric stress from a 3x3 Gaussian matrix W: off-diagonal (W_ij+W_ji)/sqrt2,
-   diagonal sqrt2*W_ii - trace(sqrt2*W)/3 (deviatoric, FDT-consistent), plus
-   the bulk part zeta_d*tr(W); all scaled by eta_d = eta_sd/sqrt(dV*dt).
+/* Symmetric stochastic stress with the Landau-Lifshitz covariance
+   <S_ij S_kl> = ae^2 (d_ik d_jl + d_il d_jk - 2/3 d_ij d_kl) + 3 az^2 d_ij d_kl.
+   random_stresses_pnts_r symmetrises a 3x3 Gaussian matrix W: off-diagonals
+   (W_ij+W_ji)/sqrt2 are N(0,1), so one normal each is equal in law; the
+   diagonal here is sqrt2*W_ii - trace/3 (variance 4/3, the deviatoric value
+   the covariance above needs; the reference uses W_ii - trace(W)/3), plus the
+   bulk part az*tr(W).  Six normals per point from two Philox blocks:
+   g0..g2 diagonal, g3..g5 the xy, xz, yz off-diagonals.
    stream 1 = x faces, 2 = y faces, 3 = z faces, 4 = volume points;
-   index = point*N + linear cell/face id. */
+   id = point*N + linear cell/face id, counters 2*id and 2*id+1. */
 static void noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta,
                          double S[6]) {
-    double g[12];
-    orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 3 * id, g);
-    orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 3 * id + 1, g + 4);
-    /* g[0..8] used as W row-major */
-    const double s2 = 1.4142135623730951, s2i = 0.7071067811865476;
-    double dxx = s2 * g[0], dyy = s2 * g[4], dzz = s2 * g[8];
+    double g[8];
+    orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 2 * id, g);
+    orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 2 * id + 1, g + 4);
+    const double s2 = 1.4142135623730951;
+    double dxx = s2 * g[0], dyy = s2 * g[1], dzz = s2 * g[2];
     double tr = (dxx + dyy + dzz) / 3.0;
-    double trW = g[0] + g[4] + g[8];
+    double trW = g[0] + g[1] + g[2];
     S[0] = amp_eta * (dxx - tr) + amp_zeta * trW;
     S[1] = amp_eta * (dyy - tr) + amp_zeta * trW;
     S[2] = amp_eta * (dzz - tr) + amp_zeta * trW;
-    S[3] = amp_eta * s2i * (g[1] + g[3]);
-    S[4] = amp_eta * s2i * (g[2] + g[6]);
-    S[5] = amp_eta * s2i * (g[5] + g[7]);
+    S[3] = amp_eta * g[3];
+    S[4] = amp_eta * g[4];
+    S[5] = amp_eta * g[5];
+}
+
+int orc_noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta, double S[6]) {
+    noise_stress(c, stream, id, amp_eta, amp_zeta, S);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -858,24 +868,37 @@ static void inner_integral(orc_ctx *c, const double *q, double noise_amp_eta, do
             }
 }
 
-static void noise_amps(const orc_ctx *c, double dt, double *ae, double *az) {
+/* Point amplitudes of the stochastic stress.  sqrt(dVi/dt) is sqrt_dVdt_i of
+   src/old_mod/perseushd_main.f90:128-131.  A Gauss-weighted average of
+   independent point values has variance var*sum_p (w_p/W)^2, so the point
+   amplitude carries kappa = 1/sqrt(sum_p (w_p/W)^2) for the face- (or
+   cell-) averaged stress to have the Landau-Lifshitz variance
+   eta_sd^2/(dV dt); face points use the 2D weights, volume points the 3D. */
+static void noise_amps(const orc_ctx *c, double dt, int vol, double *ae, double *az) {
     *ae = 0; *az = 0;
     if (!c->c.noise) return;
     double dVi = c->dxi * c->dyi * c->dzi;
-    double sq = sqrt(dVi / dt); /* sqrt_dVdt_i (src/old_mod/perseushd_main.f90:128-131) */
-    *ae = c->c.eta_sd * sq;
-    *az = c->c.zeta_sd * sq;
+    double sq = sqrt(dVi / dt);
+    int n = vol ? c->npg : c->nface;
+    const double *w = vol ? c->w3 : c->w2;
+    double W = 0, s2 = 0;
+    for (int p = 0; p < n; p++) W += w[p];
+    for (int p = 0; p < n; p++) s2 += (w[p] / W) * (w[p] / W);
+    double kappa = 1.0 / sqrt(s2);
+    *ae = c->c.eta_sd * sq * kappa;
+    *az = c->c.zeta_sd * sq * kappa;
 }
 
 /* glflux (src/flux.f90:932-1022) */
 int orc_glflux(orc_ctx *c, const double *q, double *glflux) {
     const orc_config *g = &c->c;
     const int nb = c->nb, nface = c->nface, nQ = c->nQ;
-    double ae, az;
-    noise_amps(c, c->noise_dt, &ae, &az);
+    double ae, az, aev, azv;
+    noise_amps(c, c->noise_dt, 0, &ae, &az);
+    noise_amps(c, c->noise_dt, 1, &aev, &azv);
     exchange_and_bc(c, q);
     calc_face_fluxes(c, q, ae, az);
-    inner_integral(c, q, ae, az);
+    inner_integral(c, q, aev, azv);
     const double *w2 = c->w2;
     for (int f = 0; f < nQ; f++)
         for (int k = 0; k < g->nz; k++)
@@ -950,8 +973,10 @@ int orc_euler_step(orc_ctx *c, double *q_in, double *q_out, double dt) {
     const orc_config *g = &c->c;
     const int nb = c->nb, nQ = c->nQ;
     if (g->limiter) orc_limiter(c, q_in);
+    const double keep_dt = c->noise_dt; /* the stage's dt sets the noise amplitude */
     c->noise_dt = dt;
     orc_glflux(c, q_in, c->glf);
+    c->noise_dt = keep_dt;
     orc_source(c, q_in, c->src);
     double faci = 1. / (1. + dt * g->coll);
     long n = c->ncell;
@@ -1009,6 +1034,7 @@ int orc_update(orc_ctx *c, double *q_io, double *q1, double *q2, double dt) {
         if ((rc = orc_euler_step(c, q1, q2, dt))) return rc;
         for (long e = 0; e < n; e++) q_io[e] = 0.5 * (q_io[e] + q2[e]);
     }
+    c->step_index++; /* the noise of the next step is a fresh draw */
     return 0;
 }
 

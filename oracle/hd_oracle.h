@@ -44,7 +44,8 @@ typedef struct orc_ctx orc_ctx;
 
 orc_ctx *orc_create(const orc_config *cfg);
 void orc_destroy(orc_ctx *c);
-int orc_set_step_index(orc_ctx *c, uint64_t step);    /* noise counter */
+int orc_set_step_index(orc_ctx *c, uint64_t step);    /* noise counter; orc_update advances it by one */
+int orc_set_noise_dt(orc_ctx *c, double dt);          /* dt of the noise amplitude in orc_rhs (default 1) */
 
 /* tables (for pinning tests); arrays sized by the caller */
 int orc_tables(const orc_ctx *c, double *xq, double *w1, double *w2, double *w3,
@@ -71,6 +72,8 @@ double orc_cfcal(const orc_ctx *c, const double *q, int dir);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_noise_normals(uint64_t seed, uint32_t realization, uint64_t step, uint32_t stream,
                        uint64_t index, double out[4]);
+/* stochastic stress (xx,yy,zz,xy,xz,yz) of point id in a stream at the current step index */
+int orc_noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta, double S[6]);
 
 const char *orc_last_error(void);
 int64_t orc_last_bad_cell(void);

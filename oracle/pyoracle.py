@@ -68,8 +68,10 @@ def lib():
         L.orc_cfcal.restype = C.c_double
         L.orc_tables.argtypes = [C.c_void_p] + [dp] * 13
         L.orc_set_step_index.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_set_noise_dt.argtypes = [C.c_void_p, C.c_double]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, dp]
+        L.orc_noise_stress.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double, dp]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_bad_cell.restype = C.c_int64
         _lib = L
@@ -144,6 +146,19 @@ class Oracle:
 
     def set_step_index(self, n: int):
         lib().orc_set_step_index(self.h, int(n))
+
+    def set_noise_dt(self, dt: float):
+        lib().orc_set_noise_dt(self.h, float(dt))
+
+    def noise_stress(self, stream: int, ids, amp_eta: float, amp_zeta: float) -> np.ndarray:
+        """(len(ids), 6) stresses xx,yy,zz,xy,xz,yz at the current step index."""
+        ids = np.asarray(ids, dtype=np.uint64).ravel()
+        out = np.zeros((ids.size, 6))
+        row = np.zeros(6)
+        for n, i in enumerate(ids):
+            lib().orc_noise_stress(self.h, int(stream), int(i), float(amp_eta), float(amp_zeta), _p(row))
+            out[n] = row
+        return out
 
     def flux_points(self, qp: np.ndarray, direction: int) -> np.ndarray:
         qp = np.ascontiguousarray(qp, dtype=np.float64)

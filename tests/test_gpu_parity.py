@@ -196,3 +196,75 @@ def test_reference_front_end(torch_mod, oracle_mod):
     assert float(t) == pytest.approx(float(to), rel=1e-13)
     assert int(nout) >= 1
     assert field_rel_err(np.ascontiguousarray(Q.T), q) < TOL
+
+
+# ------------------------------------------------------- stochastic stress ----
+def _noisy(nb, n, **kw):
+    q = 3 if nb == 10 else 2
+    base = dict(nx=n[0], ny=n[1], nz=n[2], nbasis=nb, iquad=q, nQ=11, ivis=1, coll=20.0, colvis=0.5,
+                noise=True, eta_sd=0.02, zeta_sd=0.005, lx=float(n[0]), ly=float(n[1]), lz=float(n[2]),
+                T_floor=1e-4, seed=2024, realization=3)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+NOISE_CASES = {
+    "noise_nb4_hllc_periodic": _noisy(4, (8, 6, 5), riemann="hllc"),
+    "noise_nb10_roe_walls": _noisy(10, (6, 5, 4), riemann="roe",
+                                   bc=("periodic", "periodic", "noslip", "noslip", "wall", "wall")),
+    "noise_nb8_central_outflow_ivis0": _noisy(8, (5, 6, 4), riemann="central", ivis=0, bc=("outflow",) * 6),
+}
+
+
+@pytest.mark.parametrize("name", list(NOISE_CASES))
+def test_noise_rhs_parity(torch_mod, oracle_mod, name):
+    """Same (seed, realization, step) key: the HIP field matches the oracle's draw."""
+    from hermeshd_amd.solver import Solver
+    cfg = NOISE_CASES[name]
+    print(f"[noise_rhs_parity] {name} {cfg.shape}", flush=True)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = smooth_state(cfg, seed=5)
+    s = Solver(cfg)
+    qd = _dev(q)
+    assert field_rel_err(s.rhs(qd).cpu().numpy(), o.rhs(q)) < TOL
+    s.set_step(12345)
+    o.set_step_index(12345)
+    r = s.rhs(qd).cpu().numpy()
+    ref = o.rhs(q)
+    assert field_rel_err(r, ref) < TOL
+    r0 = oracle_mod.Oracle(cfg.replace(noise=False).as_ints()).rhs(q)
+    assert np.abs(ref - r0).max() > 1e-4            # the noise is present
+
+
+@pytest.mark.parametrize("name,nsteps", [("noise_nb4_hllc_periodic", 6), ("noise_nb10_roe_walls", 3),
+                                         ("noise_nb8_central_outflow_ivis0", 3)])
+def test_noise_step_parity(torch_mod, oracle_mod, name, nsteps):
+    """Reference step semantics with noise: a fresh field per step, shared by the stages."""
+    from hermeshd_amd.solver import Solver
+    cfg = NOISE_CASES[name]
+    print(f"[noise_step_parity] {name} x{nsteps}", flush=True)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = smooth_state(cfg, seed=6)
+    qd = _dev(q)
+    q1d = torch_mod.zeros_like(qd)
+    s = Solver(cfg)
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    t, dt = np.array(0.0), np.array(0.0)
+    for n in range(nsteps):
+        o.step(q, q1, q2, t, dt)
+        s.step(qd, q1d)
+    s.check()
+    assert field_rel_err(qd.cpu().numpy(), q) < TOL
+
+
+def test_noise_euler_step_parity(torch_mod, oracle_mod):
+    from hermeshd_amd.solver import Solver
+    cfg = NOISE_CASES["noise_nb4_hllc_periodic"]
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = smooth_state(cfg, seed=7)
+    out = np.zeros_like(q)
+    qd = _dev(q); outd = torch_mod.zeros_like(qd)
+    s = Solver(cfg)
+    s.euler_step(qd, outd, 2e-3)
+    o.euler_step(q, out, 2e-3)
+    assert field_rel_err(outd.cpu().numpy(), out) < TOL

@@ -363,3 +363,122 @@ def test_noise_normals_moments(oracle_mod):
     vals = np.concatenate([oracle_mod.noise_normals(1234, 3, 7, 1, i) for i in range(20000)])
     assert abs(vals.mean()) < 0.02 and abs(vals.std() - 1) < 0.02
     assert abs(np.mean(vals ** 4) - 3) < 0.15
+
+
+# ---------------------------------------------------------------- noise ----
+def _noise_cfg(nb=4, n=(8, 8, 8), **kw):
+    q = 3 if nb == 10 else 2
+    base = dict(nx=n[0], ny=n[1], nz=n[2], nbasis=nb, iquad=q, nQ=11, ivis=1, coll=10.0, colvis=1.0,
+                riemann="hllc", noise=True, eta_sd=0.05, zeta_sd=0.0, lx=float(n[0]), ly=float(n[1]),
+                lz=float(n[2]), T_floor=1e-4, limiter=False, seed=99)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+def test_noise_stress_covariance(oracle_mod):
+    """Landau-Lifshitz covariance <S_ij S_kl> = ae^2 (d_ik d_jl + d_il d_jk - 2/3 d_ij d_kl)
+    + 3 az^2 d_ij d_kl, zero mean, traceless without bulk noise."""
+    o = oracle_mod.Oracle(_noise_cfg().as_ints())
+    n = 40000
+    S = o.noise_stress(1, np.arange(n), 1.0, 0.0)
+    assert np.abs(S[:, :3].sum(axis=1)).max() < 1e-13        # deviatoric: exactly traceless
+    C = np.cov(S.T)
+    want = np.diag([4 / 3, 4 / 3, 4 / 3, 1, 1, 1])
+    want[0, 1] = want[1, 0] = want[0, 2] = want[2, 0] = want[1, 2] = want[2, 1] = -2 / 3
+    assert np.abs(C - want).max() < 0.04
+    assert np.abs(S.mean(axis=0)).max() < 0.02
+    Sb = o.noise_stress(4, np.arange(n), 0.0, 1.0)            # bulk part only: az * tr(W)
+    assert np.allclose(Sb[:, 0], Sb[:, 1]) and np.allclose(Sb[:, 0], Sb[:, 2])
+    assert abs(Sb[:, 0].var() - 3.0) < 0.1 and np.abs(Sb[:, 3:]).max() == 0.0
+    # a stream and an id select independent draws
+    S2 = o.noise_stress(2, np.arange(n), 1.0, 0.0)
+    assert abs(np.corrcoef(S[:, 3], S2[:, 3])[0, 1]) < 0.03
+
+
+def test_noise_conserves_totals_periodic(oracle_mod):
+    """One draw per face point shared by both cells: with periodic boundaries the
+    noisy fluxes cancel in the domain totals of every conserved field."""
+    for nb in (4, 10):
+        cfg = _noise_cfg(nb=nb, n=(5, 4, 3), eta_sd=0.3)
+        o = oracle_mod.Oracle(cfg.as_ints())
+        q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.2, -0.1, 0.3))
+        rng = np.random.default_rng(3)
+        q[1:, :5] = 0.02 * rng.standard_normal(q[1:, :5].shape)
+        r = o.rhs(q)
+        r0 = oracle_mod.Oracle(cfg.replace(noise=False).as_ints()).rhs(q)
+        assert np.abs(r - r0).max() > 1e-3                    # the noise is there
+        tot = r[0].reshape(cfg.nQ, -1).sum(axis=1)
+        assert np.abs(tot[:5]).max() < 1e-11 * np.abs(r[0]).max()
+
+
+def test_noise_affine_in_amplitude(oracle_mod):
+    """The stress enters the fluxes linearly and no solver branch depends on it,
+    so rhs(2a) - 2 rhs(a) + rhs(0) = 0."""
+    cfg = _noise_cfg(nb=10, n=(4, 3, 3), riemann="hllc")
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.3, 0.1, -0.2))
+    rng = np.random.default_rng(8)
+    q[1:, :5] = 0.01 * rng.standard_normal(q[1:, :5].shape)
+    rs = [oracle_mod.Oracle(cfg.replace(eta_sd=a).as_ints()).rhs(q) for a in (0.0, 0.1, 0.2)]
+    assert np.abs(rs[2] - 2 * rs[1] + rs[0]).max() < 1e-12 * np.abs(rs[1] - rs[0]).max() * 10
+
+
+@pytest.mark.parametrize("nb", [4, 10])
+def test_noise_face_average_variance(oracle_mod, nb):
+    """Fluid at rest: mode 0 of the momentum rhs is the divergence of the
+    face-averaged stress, whose variance must be eta_sd^2/(dV dt) (the
+    Landau-Lifshitz amplitude).  Var(rhs_mx) = 2 eta^2/(dV dt) (4/3 dxi^2 + dyi^2 + dzi^2)."""
+    n = (16, 16, 16) if nb == 4 else (10, 10, 10)
+    cfg = _noise_cfg(nb=nb, n=n, eta_sd=0.2, lx=8.0, ly=16.0, lz=12.0)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    dt = 0.01
+    o.set_noise_dt(dt)
+    q = ics.uniform(cfg, rho=1.0, T=1.0)
+    dxi, dyi, dzi = n[0] / 8.0, n[1] / 16.0, n[2] / 12.0
+    dVi = dxi * dyi * dzi
+    want = 2 * cfg.eta_sd ** 2 * dVi / dt * (4 / 3 * dxi ** 2 + dyi ** 2 + dzi ** 2)
+    samples = []
+    for step in range(6 if nb == 4 else 10):
+        o.set_step_index(step)
+        samples.append(o.rhs(q)[0, 1].ravel())
+    got = np.concatenate(samples).var()
+    assert abs(got / want - 1) < 0.08
+
+
+def test_noise_volume_amplitude(oracle_mod):
+    """Mode x of the x-momentum rhs (nb 4, two-point rule) collects the volume
+    points and the x, y, z faces with independent draws:
+    Var = (3/4)^2 eta^2 dVi/dt [64*4/3 dxi^2 + 128/3 dxi^2 + 32/3 (dyi^2 + dzi^2)]."""
+    n = (16, 16, 16)
+    cfg = _noise_cfg(nb=4, n=n, eta_sd=0.2, lx=16.0, ly=8.0, lz=32.0)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    dt = 0.02
+    o.set_noise_dt(dt)
+    q = ics.uniform(cfg, rho=1.0, T=1.0)
+    dxi, dyi, dzi = 1.0, 2.0, 0.5
+    want = 0.5625 * cfg.eta_sd ** 2 * dxi * dyi * dzi / dt * (
+        64 * 4 / 3 * dxi ** 2 + 128 / 3 * dxi ** 2 + 32 / 3 * (dyi ** 2 + dzi ** 2))
+    samples = []
+    for step in range(6):
+        o.set_step_index(step)
+        samples.append(o.rhs(q)[1, 1].ravel())
+    got = np.concatenate(samples).var()
+    assert abs(got / want - 1) < 0.08
+
+
+def test_noise_keyed_by_step(oracle_mod):
+    """The draw depends on (seed, realization, step): an update advances the step
+    index; the same key reproduces the same field."""
+    cfg = _noise_cfg(nb=4, n=(4, 4, 4), eta_sd=0.1)
+    q = ics.uniform(cfg, rho=1.0, T=1.0)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    a = o.rhs(q)
+    assert np.array_equal(a, o.rhs(q))
+    o.update(q.copy(), np.zeros_like(q), np.zeros_like(q), 1e-3)
+    b = o.rhs(q)
+    assert not np.allclose(a, b)
+    o.set_step_index(0)
+    assert np.array_equal(a, o.rhs(q))
+    o2 = oracle_mod.Oracle(cfg.replace(realization=1).as_ints())
+    assert not np.allclose(a, o2.rhs(q))
+    o3 = oracle_mod.Oracle(cfg.replace(seed=100).as_ints())
+    assert not np.allclose(a, o3.rhs(q))
