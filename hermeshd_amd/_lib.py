@@ -19,7 +19,13 @@ EXPORTED = [
     "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_update",
     "hd_step", "hd_set_step", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
     "hd_last_error",
+    "hd_ens_create", "hd_ens_destroy", "hd_ens_state_size", "hd_ens_offsets", "hd_ens_device_bytes",
+    "hd_ens_limit", "hd_ens_rhs", "hd_ens_update", "hd_ens_run", "hd_ens_set_step", "hd_ens_check",
+    "hd_ens_launch_count", "hd_ens_launch_info", "hd_ens_set_launch", "hd_ens_get_launch", "hd_ens_tune",
 ]
+# entry points whose return value is not an error code
+NON_RC = ("hd_device_bytes", "hd_last_error", "hd_ens_state_size", "hd_ens_device_bytes", "hd_ens_launch_count",
+          "hd_ens_get_launch")
 
 
 class HDConfigC(C.Structure):
@@ -74,8 +80,30 @@ def lib() -> C.CDLL:
     L.hd_device_bytes.argtypes = [vp]
     L.hd_device_bytes.restype = C.c_int64
     L.hd_last_error.restype = C.c_char_p
+    i64p = C.POINTER(C.c_int64)
+    L.hd_ens_create.argtypes = [C.POINTER(HDConfigC), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_void_p)]
+    L.hd_ens_destroy.argtypes = [vp]
+    L.hd_ens_state_size.argtypes = [vp]
+    L.hd_ens_state_size.restype = C.c_int64
+    L.hd_ens_offsets.argtypes = [vp, i64p]
+    L.hd_ens_device_bytes.argtypes = [vp]
+    L.hd_ens_device_bytes.restype = C.c_int64
+    L.hd_ens_limit.argtypes = [vp, vp, vp]
+    L.hd_ens_rhs.argtypes = [vp, vp, vp, vp]
+    L.hd_ens_update.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hd_ens_run.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
+    L.hd_ens_set_step.argtypes = [vp, C.c_uint64, vp]
+    L.hd_ens_check.argtypes = [vp, vp, i64p, i64p]
+    L.hd_ens_launch_count.argtypes = []
+    L.hd_ens_launch_count.restype = C.c_int32
+    L.hd_ens_launch_info.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
+    L.hd_ens_set_launch.argtypes = [vp, C.c_int32]
+    L.hd_ens_get_launch.argtypes = [vp]
+    L.hd_ens_get_launch.restype = C.c_int
+    L.hd_ens_tune.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.POINTER(C.c_int32), dp]
     for name in EXPORTED:
-        if name not in ("hd_device_bytes", "hd_last_error"):
+        if name not in NON_RC:
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

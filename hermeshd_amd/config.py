@@ -164,3 +164,22 @@ def poiseuille_2d(nx: int = 64, ny: int = 128, vis: float = 0.01) -> HDConfig:
                     integrator="heun", bc=("periodic", "periodic", "noslip", "noslip", "periodic", "periodic"),
                     lx=float(nx), ly=float(ny), lz=1.0, gamma=5.0 / 3.0, T_floor=1.0, coll=coll,
                     colvis=coll * vis, body_force=(1e-3, 0.0, 0.0))
+
+
+def fluct_ensemble(vis: float = 0.3, kB: float = 1e-4, sizes=((96, 16), (96, 24), (64, 32)),
+                   seed: int = 1234):
+    """Fluctuating-hydrodynamics ensemble: periodic boxes of unit cells at
+    rho = T = 1, v = 0; 10-moment stress relaxation with coll = rho*T/vis,
+    colvis = coll*vis (src/initialcon.f90:258-259) giving viscosity vis;
+    stochastic stress with the Landau-Lifshitz amplitude eta_sd =
+    sqrt(2 vis kB T) for a fluctuation temperature kB*T (kB << 1 keeps the
+    fluctuations linear); linear basis, HLLC, SSP-RK2, limiter off.
+    `sizes` lists (count, n) for count boxes of n^3 cells.
+    Returns (base configuration, list of box extents)."""
+    coll = 1.0 / vis
+    base = HDConfig(nx=16, ny=16, nz=16, nbasis=4, iquad=2, nQ=11, ivis=0, riemann="hllc", limiter=False,
+                    integrator="heun", bc=("periodic",) * 6, lx=16.0, ly=16.0, lz=16.0, gamma=1.4,
+                    T_floor=1e-4, coll=coll, colvis=coll * vis, noise=True,
+                    eta_sd=math.sqrt(2.0 * vis * kB), zeta_sd=0.0, seed=seed, realization=0)
+    dims = [(n, n, n) for count, n in sizes for _ in range(count)]
+    return base, dims

@@ -16,35 +16,22 @@
 #include "hd_ctx.h"
 #include "hd_noise.h"
 
-namespace {
-
+namespace hd {
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+}  // namespace hd
 
-#define HIPCHK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(HD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-}  // namespace
-
-
-namespace {
-
-using namespace hd;
-
+namespace hd {
 // dispatch over (nbasis, iquad) x nQ x Riemann solver
-bool pick_ops(const hd_ctx* c, Ops* o) {
-    if (c->nb == 4 && c->q == 2) return hd_pick_4(c->nQ, c->cfg.riemann, o);
-    if (c->nb == 8 && c->q == 2) return hd_pick_8(c->nQ, c->cfg.riemann, o);
-    if (c->nb == 10 && c->q == 3 && c->nQ == 5) return hd_pick_10_5(c->cfg.riemann, o);
-    if (c->nb == 10 && c->q == 3 && c->nQ == 11) return hd_pick_10_11(c->cfg.riemann, o);
+bool pick_ops_for(int nb, int q, int nq, int rm, Ops* o) {
+    if (nb == 4 && q == 2) return hd_pick_4(nq, rm, o);
+    if (nb == 8 && q == 2) return hd_pick_8(nq, rm, o);
+    if (nb == 10 && q == 3 && nq == 5) return hd_pick_10_5(rm, o);
+    if (nb == 10 && q == 3 && nq == 11) return hd_pick_10_11(rm, o);
     return false;
 }
 
@@ -71,6 +58,24 @@ int validate(const hd_config* c) {
     if (!(c->gamma > 1.0)) return fail(HD_ERR_INVALID, "gamma must exceed 1");
     return HD_OK;
 }
+
+// kappa = 1/sqrt(sum_p (w_p/W)^2) of a product rule in 2 (face) or 3 (volume) dimensions
+double rule_kappa(const Tables& t, int dims) {
+    const int n = dims == 2 ? t.nface : t.npg;
+    const double* w = dims == 2 ? t.w2 : t.w3;
+    double W = 0., s2 = 0.;
+    for (int p = 0; p < n; p++) W += w[p];
+    for (int p = 0; p < n; p++) s2 += (w[p] / W) * (w[p] / W);
+    return 1.0 / std::sqrt(s2);
+}
+
+}  // namespace hd
+
+namespace {
+
+using namespace hd;
+
+bool pick_ops(const hd_ctx* c, Ops* o) { return pick_ops_for(c->nb, c->q, c->nQ, c->cfg.riemann, o); }
 
 int alloc(hd_ctx* c, void** p, size_t bytes) {
     HIPCHK(hipMalloc(p, bytes));
@@ -126,16 +131,6 @@ int advance_step(hd_ctx* c, hipStream_t s) {
     hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, c->step_ctr, 0ull, 1);
     HIPCHK(hipGetLastError());
     return HD_OK;
-}
-
-// kappa = 1/sqrt(sum_p (w_p/W)^2) of a product rule in 2 (face) or 3 (volume) dimensions
-double rule_kappa(const Tables& t, int dims) {
-    const int n = dims == 2 ? t.nface : t.npg;
-    const double* w = dims == 2 ? t.w2 : t.w3;
-    double W = 0., s2 = 0.;
-    for (int p = 0; p < n; p++) W += w[p];
-    for (int p = 0; p < n; p++) s2 += (w[p] / W) * (w[p] / W);
-    return 1.0 / std::sqrt(s2);
 }
 
 hipEvent_t next_event(hd_ctx* c) {

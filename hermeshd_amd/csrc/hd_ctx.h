@@ -9,11 +9,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <utility>
 #include <vector>
 
 #include "../../include/hermeshd_amd.h"
-#include "hd_generic.h"
+#include "hd_ensemble.h"
 
 struct hd_ctx {
     hd_config cfg;
@@ -47,6 +48,40 @@ struct hd_ctx {
     size_t ev_used = 0;
 };
 
+// An ensemble: realizations of one configuration on boxes of different
+// sizes, packed back to back (hd_ens_* entry points).
+struct hd_ens {
+    hd_config base;
+    int nreal = 0, q = 0, nb = 0, nQ = 0, nface = 0, npg = 0;
+    hd::Phys ph;
+    hd::LimParams lp;
+    double dxi = 0., dyi = 0., dzi = 0., kap_f = 0., kap_v = 0.;
+    std::vector<hd::ens::EnsReal> hreal;
+    std::vector<long> hpts;             // noise point prefix sums (nreal+1)
+    std::vector<int2> hwork;
+    hd::ens::EnsReal* dreal = nullptr;
+    long* dpts = nullptr;
+    int2* dwork = nullptr;
+    size_t work_cap = 0;
+    long state_doubles = 0, cells = 0, noise_doubles = 0;
+    double* noise = nullptr;
+    double* one_dt = nullptr;          // 1.0: the noise time step of hd_ens_rhs
+    double* stage_buf = nullptr;       // RK3 third array when the caller passes none
+    unsigned long long* step_ctr = nullptr;
+    int* err = nullptr;
+    unsigned long long* bad = nullptr;
+    // launch shape (hd_ens_set_launch / hd_ens_tune)
+    int choice = -1, bs = 256, grid = 0, num_cus = 0;
+    hd::ens::Launch launch{};
+    // captured step (hd_ens_run)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void* gkey[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int gchoice = -1;
+    int64_t bytes = 0;
+    int device = 0;
+};
+
 namespace hd {
 
 struct Ops {
@@ -54,8 +89,26 @@ struct Ops {
     void (*faces)(hd_ctx*, const double*, hipStream_t);
     void (*cell)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
     void (*fused)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
+    void (*ens_limit)(hd_ens*, double*, hipStream_t);
+    void (*ens_stage)(hd_ens*, const double*, const double*, double*, const StageParams&, hipStream_t);
 };
 
+// error text of the calling thread (hd_last_error) and the error return
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return hd::fail(HD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+// checks a configuration (the per-realization extents of an ensemble included)
+int validate(const hd_config* c);
+// 1/sqrt(sum_p (w_p/W)^2) of the 2-D (face) or 3-D (volume) product rule
+double rule_kappa(const Tables& t, int dims);
+
+// dispatch table of one (nbasis, iquad, nQ, Riemann solver); false if not built
+bool pick_ops_for(int nb, int q, int nq, int rm, Ops* o);
 bool hd_pick_4(int nq, int rm, Ops* o);
 bool hd_pick_8(int nq, int rm, Ops* o);
 bool hd_pick_10_5(int rm, Ops* o);

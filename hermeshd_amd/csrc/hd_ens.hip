@@ -1,0 +1,445 @@
+// hd_ens.hip -- C ABI of the ensemble: many realizations of one configuration
+// on boxes of different sizes, advanced together.
+//
+// Each realization is what one run of the reference computes (the time loop
+// of src/main.f90:22-49 over step, src/hermeshd.f90:53-62, with the noise of
+// src/random.f90): its own box, its own noise stream.  Here all of them share
+// one launch per RK stage, one noise launch per step, and a captured HIP
+// graph of the whole step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hd_ctx.h"
+
+namespace {
+
+using namespace hd;
+
+struct Candidate {
+    int bs, bx, by, bz, linear, wg_per_cu, xcd;
+};
+
+// Launch shapes the tuner times: runs of consecutive cells or bricks, plain
+// grids (wg_per_cu 0: one workgroup per tile) or persistent grids.
+const Candidate kCandidates[] = {
+    {256, 256, 1, 1, 1, 0, 0}, {256, 256, 1, 1, 1, 1, 1}, {256, 256, 1, 1, 1, 2, 1}, {256, 256, 1, 1, 1, 2, 0},
+    {128, 128, 1, 1, 1, 0, 0}, {128, 128, 1, 1, 1, 2, 1}, {64, 64, 1, 1, 1, 0, 0},    {64, 64, 1, 1, 1, 4, 1},
+    {256, 16, 4, 4, 0, 0, 0},  {256, 16, 4, 4, 0, 1, 1},  {256, 8, 8, 4, 0, 0, 0},    {256, 8, 8, 4, 0, 1, 1},
+    {128, 8, 4, 4, 0, 0, 0},   {128, 8, 4, 4, 0, 2, 1},   {64, 4, 4, 4, 0, 0, 0},     {64, 4, 4, 4, 0, 4, 1},
+};
+constexpr int kNumCandidates = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+
+int ens_alloc(hd_ens* e, void** p, size_t bytes) {
+    HIPCHK(hipMalloc(p, bytes));
+    e->bytes += (int64_t)bytes;
+    return HD_OK;
+}
+
+void drop_graph(hd_ens* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+    e->gchoice = -1;
+}
+
+// Tile counts and the work list of a launch shape.
+int set_launch(hd_ens* e, int choice) {
+    if (choice < 0 || choice >= kNumCandidates) return fail(HD_ERR_INVALID, "launch choice out of range");
+    const Candidate& k = kCandidates[choice];
+    e->hwork.clear();
+    for (int r = 0; r < e->nreal; r++) {
+        ens::EnsReal& R = e->hreal[r];
+        long ntile;
+        if (k.linear) {
+            R.ntx = R.nty = R.ntz = 0;
+            ntile = (R.g.N + k.bs - 1) / k.bs;
+        } else {
+            R.ntx = (R.g.nx + k.bx - 1) / k.bx;
+            R.nty = (R.g.ny + k.by - 1) / k.by;
+            R.ntz = (R.g.nz + k.bz - 1) / k.bz;
+            ntile = (long)R.ntx * R.nty * R.ntz;
+        }
+        for (long t = 0; t < ntile; t++) e->hwork.push_back(make_int2(r, (int)t));
+    }
+    if (e->hwork.size() > (size_t)0x7fffffff) return fail(HD_ERR_INVALID, "work list too long");
+    // launches in flight may still read the old tables
+    HIPCHK(hipDeviceSynchronize());
+    if (e->hwork.size() > e->work_cap) {
+        if (e->dwork) (void)hipFree(e->dwork);
+        e->dwork = nullptr;
+        int rc = ens_alloc(e, (void**)&e->dwork, e->hwork.size() * sizeof(int2));
+        if (rc) return rc;
+        e->work_cap = e->hwork.size();
+    }
+    HIPCHK(hipMemcpy(e->dwork, e->hwork.data(), e->hwork.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dreal, e->hreal.data(), e->hreal.size() * sizeof(ens::EnsReal), hipMemcpyHostToDevice));
+    e->launch.bx = k.bx; e->launch.by = k.by; e->launch.bz = k.bz;
+    e->launch.linear = k.linear;
+    e->launch.nwork = (int)e->hwork.size();
+    e->bs = k.bs;
+    if (k.wg_per_cu == 0) {
+        e->grid = e->launch.nwork;
+        e->launch.xcd_chunk = 0;
+    } else {
+        // persistent: a multiple of 8 workgroups, at most one per work item
+        // rounded up to 8 (workgroups without items return at once)
+        long g = (long)k.wg_per_cu * e->num_cus;
+        g = std::min<long>(g, ((long)e->launch.nwork + 7) / 8 * 8);
+        e->grid = (int)std::max<long>(8, g / 8 * 8);
+        e->launch.xcd_chunk = k.xcd;
+    }
+    e->choice = choice;
+    drop_graph(e);
+    return HD_OK;
+}
+
+StageParams ens_params(const hd_ens* e, const double* dt_dev, int combine, int mode) {
+    StageParams sp;
+    sp.dt = dt_dev;
+    sp.coll = e->base.coll;
+    sp.ivis = e->base.ivis;
+    sp.combine = combine;
+    sp.mode = mode;
+    for (int d = 0; d < 3; d++) sp.bf[d] = e->base.body_force[d];
+    sp.err = e->err;
+    sp.bad = e->bad;
+    sp.limit_out = 0;
+    sp.lp = e->lp;
+    for (int d = 0; d < 4; d++) sp.noise[d] = nullptr;  // set per realization by the kernel
+    sp.nfb = 0;
+    return sp;
+}
+
+int ens_noise(hd_ens* e, const double* dt_dev, hipStream_t s) {
+    if (!e->base.noise) return HD_OK;
+    ens::EnsNoise np;
+    np.seed = e->base.seed;
+    np.step = e->step_ctr;
+    np.dt = dt_dev;
+    np.eta_sd = e->base.eta_sd;
+    np.zeta_sd = e->base.zeta_sd;
+    np.dVi = e->dxi * e->dyi * e->dzi;
+    np.kap_f = e->kap_f;
+    np.kap_v = e->kap_v;
+    np.nface = e->nface;
+    np.npg = e->npg;
+    np.nreal = e->nreal;
+    np.pts = e->dpts;
+    np.reals = e->dreal;
+    np.out = e->noise;
+    const long total = e->hpts[e->nreal];
+    hipLaunchKernelGGL(ens::k_ens_noise, dim3(blocks_for(total, 256)), dim3(256), 0, s, np);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int ens_stage(hd_ens* e, const Ops& o, double* qin, const double* qn, double* qout, const double* dt_dev,
+              int combine, hipStream_t s, bool in_limited, bool limit_out) {
+    if (e->base.limiter && !in_limited) o.ens_limit(e, qin, s);
+    StageParams sp = ens_params(e, dt_dev, combine, 0);
+    sp.limit_out = (limit_out && e->base.limiter) ? 1 : 0;
+    o.ens_stage(e, qin, qn, qout, sp, s);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+// One step, stream-ordered: noise, stages with fused SSP combinations, step index.
+int ens_step(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt_dev, hipStream_t s) {
+    Ops o;
+    if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    int rc;
+    if ((rc = ens_noise(e, dt_dev, s))) return rc;
+    if (e->base.integrator == 3) {
+        double* q2b = q2 ? q2 : e->stage_buf;
+        if ((rc = ens_stage(e, o, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if ((rc = ens_stage(e, o, q1, q_io, q2b, dt_dev, 2, s, true, true))) return rc;
+        if ((rc = ens_stage(e, o, q2b, q_io, q_io, dt_dev, 3, s, true, false))) return rc;
+    } else {
+        if ((rc = ens_stage(e, o, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if ((rc = ens_stage(e, o, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
+    }
+    if (e->base.noise) {
+        hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, e->step_ctr, 0ull, 1);
+        HIPCHK(hipGetLastError());
+    }
+    return HD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, const uint32_t* realization_ids,
+                  hd_ens** out) {
+    if (!out || !base || !dims) return fail(HD_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (nreal < 1) return fail(HD_ERR_INVALID, "nreal must be >= 1");
+    int rc = validate(base);
+    if (rc) return rc;
+    for (int r = 0; r < nreal; r++) {
+        hd_config c = *base;
+        c.nx = dims[3 * r]; c.ny = dims[3 * r + 1]; c.nz = dims[3 * r + 2];
+        if ((rc = validate(&c))) return rc;
+        if ((long)c.nx * c.ny * c.nz > 0x7fffffffL) return fail(HD_ERR_INVALID, "realization too large");
+    }
+    hd_ens* e = new (std::nothrow) hd_ens();
+    if (!e) return fail(HD_ERR_INVALID, "out of host memory");
+    e->base = *base;
+    e->nreal = nreal;
+    e->q = base->iquad; e->nb = base->nbasis; e->nQ = base->nQ;
+    e->nface = e->q * e->q; e->npg = e->nface * e->q;
+    if (hipGetDevice(&e->device) != hipSuccess) { delete e; return fail(HD_ERR_HIP, "no HIP device"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) { delete e; return fail(HD_ERR_HIP, "device query failed"); }
+    e->num_cus = prop.multiProcessorCount;
+    // every realization has the cell size of the base configuration
+    e->dxi = (double)base->nx / base->lx;
+    e->dyi = (double)base->ny / base->ly;
+    e->dzi = (double)base->nz / base->lz;
+    Phys& ph = e->ph;
+    ph.gamma = base->gamma; ph.gm1 = base->gamma - 1.;
+    ph.P_floor = base->T_floor * base->rh_floor; ph.rh_floor = base->rh_floor;
+    ph.colvis = base->colvis; ph.c2d3cv = (2. / 3.) * base->colvis; ph.c4d3cv = (4. / 3.) * base->colvis;
+    e->lp.rh_floor = base->rh_floor; e->lp.epsP = base->rh_floor * base->T_floor; e->lp.gm1 = ph.gm1;
+    Tables t;
+    build_tables(e->nb, e->q, &t);
+    e->kap_f = rule_kappa(t, 2);
+    e->kap_v = rule_kappa(t, 3);
+    e->hreal.resize(nreal);
+    e->hpts.assign(nreal + 1, 0);
+    long qoff = 0, noff = 0, cell0 = 0;
+    for (int r = 0; r < nreal; r++) {
+        ens::EnsReal& R = e->hreal[r];
+        Grid& g = R.g;
+        g.nx = dims[3 * r]; g.ny = dims[3 * r + 1]; g.nz = dims[3 * r + 2]; g.nQ = e->nQ;
+        g.N = (long)g.nx * g.ny * g.nz;
+        g.dxi = e->dxi; g.dyi = e->dyi; g.dzi = e->dzi;
+        for (int s = 0; s < 6; s++) g.bc[s] = base->bc[s];
+        R.qoff = qoff;
+        R.cell0 = cell0;
+        R.nfb = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        R.noff = noff;
+        R.ntx = R.nty = R.ntz = 0;
+        R.realization = realization_ids ? realization_ids[r] : base->realization + (uint32_t)r;
+        qoff += g.N * e->nQ * e->nb;
+        cell0 += g.N;
+        if (base->noise) noff += 3L * 6 * e->nface * R.nfb + 6L * e->npg * g.N;
+        e->hpts[r + 1] = e->hpts[r] + (base->noise ? 3L * e->nface * R.nfb + (long)e->npg * g.N : 0);
+    }
+    e->state_doubles = qoff;
+    e->cells = cell0;
+    e->noise_doubles = noff;
+    rc = HD_OK;
+    if ((rc = ens_alloc(e, (void**)&e->dreal, nreal * sizeof(ens::EnsReal))) ||
+        (rc = ens_alloc(e, (void**)&e->dpts, (nreal + 1) * sizeof(long))) ||
+        (rc = ens_alloc(e, (void**)&e->one_dt, sizeof(double))) ||
+        (rc = ens_alloc(e, (void**)&e->step_ctr, sizeof(unsigned long long))) ||
+        (rc = ens_alloc(e, (void**)&e->err, sizeof(int))) ||
+        (rc = ens_alloc(e, (void**)&e->bad, sizeof(unsigned long long))) ||
+        (noff > 0 && (rc = ens_alloc(e, (void**)&e->noise, noff * sizeof(double)))) ||
+        (base->integrator == 3 && (rc = ens_alloc(e, (void**)&e->stage_buf, qoff * sizeof(double))))) {
+        std::string msg = g_last_error;
+        hd_ens_destroy(e);
+        return fail(rc, msg);
+    }
+    const double one = 1.0;
+    if (hipMemcpy(e->dpts, e->hpts.data(), (nreal + 1) * sizeof(long), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->one_dt, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(e->step_ctr, 0, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(e->err, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(e->bad, 0xff, sizeof(unsigned long long)) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        hd_ens_destroy(e);
+        return fail(HD_ERR_HIP, "initialising ensemble buffers failed");
+    }
+    if ((rc = set_launch(e, 0))) {
+        std::string msg = g_last_error;
+        hd_ens_destroy(e);
+        return fail(rc, msg);
+    }
+    *out = e;
+    return HD_OK;
+}
+
+int hd_ens_destroy(hd_ens* e) {
+    if (!e) return HD_OK;
+    drop_graph(e);
+    if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
+    void* ptrs[] = {e->dreal, e->dpts, e->dwork, e->noise, e->one_dt, e->stage_buf, e->step_ctr, e->err, e->bad};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete e;
+    return HD_OK;
+}
+
+int64_t hd_ens_state_size(const hd_ens* e) { return e ? e->state_doubles : 0; }
+int64_t hd_ens_device_bytes(const hd_ens* e) { return e ? e->bytes : 0; }
+
+int hd_ens_offsets(const hd_ens* e, int64_t* offsets) {
+    if (!e || !offsets) return fail(HD_ERR_INVALID, "NULL argument");
+    for (int r = 0; r < e->nreal; r++) offsets[r] = e->hreal[r].qoff;
+    return HD_OK;
+}
+
+int hd_ens_limit(hd_ens* e, double* q, void* stream) {
+    if (!e || !q) return fail(HD_ERR_INVALID, "NULL argument");
+    Ops o;
+    if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    o.ens_limit(e, q, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_ens_rhs(hd_ens* e, const double* q, double* rhs, void* stream) {
+    if (!e || !q || !rhs) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q == rhs) return fail(HD_ERR_INVALID, "q and rhs must differ");
+    Ops o;
+    if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ens_noise(e, e->one_dt, s);
+    if (rc) return rc;
+    o.ens_stage(e, q, q, rhs, ens_params(e, e->one_dt, 0, 1), s);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_ens_update(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt_dev, void* stream) {
+    if (!e || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q_io == q1 || (q2 && (q2 == q_io || q2 == q1))) return fail(HD_ERR_INVALID, "q_io, q1 and q2 must differ");
+    return ens_step(e, q_io, q1, q2, dt_dev, (hipStream_t)stream);
+}
+
+int hd_ens_run(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt_dev, int64_t nsteps, void* stream) {
+    if (!e || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q_io == q1 || (q2 && (q2 == q_io || q2 == q1))) return fail(HD_ERR_INVALID, "q_io, q1 and q2 must differ");
+    if (nsteps < 0) return fail(HD_ERR_INVALID, "nsteps must be >= 0");
+    const void* key[5] = {q_io, q1, q2, dt_dev, nullptr};
+    const bool same = e->gexec && e->gchoice == e->choice && std::memcmp(key, e->gkey, sizeof(key)) == 0;
+    if (!same) {
+        drop_graph(e);
+        // capture one step on the ensemble's own stream; the graph holds
+        // kernel nodes only
+        HIPCHK(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = ens_step(e, q_io, q1, q2, dt_dev, e->cap_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->cap_stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(HD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            e->gexec = nullptr;
+            return fail(HD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        std::memcpy(e->gkey, key, sizeof(key));
+        e->gchoice = e->choice;
+    }
+    for (int64_t n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(e->gexec, (hipStream_t)stream));
+    return HD_OK;
+}
+
+int hd_ens_set_step(hd_ens* e, uint64_t step, void* stream) {
+    if (!e) return fail(HD_ERR_INVALID, "NULL ensemble");
+    hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, (hipStream_t)stream, e->step_ctr,
+                       (unsigned long long)step, 0);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
+int hd_ens_check(hd_ens* e, void* stream, int64_t* bad_realization, int64_t* bad_cell) {
+    if (!e) return fail(HD_ERR_INVALID, "NULL ensemble");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    int flag = 0;
+    unsigned long long bad = ~0ull;
+    HIPCHK(hipMemcpy(&flag, e->err, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&bad, e->bad, sizeof(bad), hipMemcpyDeviceToHost));
+    int64_t br = -1, bc = -1;
+    if (flag) {
+        for (int r = 0; r < e->nreal; r++)
+            if ((long)bad >= e->hreal[r].cell0 && (long)bad < e->hreal[r].cell0 + e->hreal[r].g.N) {
+                br = r;
+                bc = (long)bad - e->hreal[r].cell0;
+            }
+        HIPCHK(hipMemset(e->err, 0, sizeof(int)));
+        HIPCHK(hipMemset(e->bad, 0xff, sizeof(unsigned long long)));
+    }
+    if (bad_realization) *bad_realization = br;
+    if (bad_cell) *bad_cell = bc;
+    if (flag)
+        return fail(HD_ERR_NONFINITE, "NaN in mode 1 of a stage output in realization " + std::to_string(br) +
+                                          " at cell " + std::to_string(bc));
+    return HD_OK;
+}
+
+int32_t hd_ens_launch_count(void) { return kNumCandidates; }
+
+int hd_ens_launch_info(int32_t choice, int32_t info[6]) {
+    if (!info || choice < 0 || choice >= kNumCandidates) return fail(HD_ERR_INVALID, "bad launch choice");
+    const Candidate& k = kCandidates[choice];
+    info[0] = k.bs; info[1] = k.bx; info[2] = k.by; info[3] = k.bz; info[4] = k.linear ? 1 : 0;
+    info[5] = k.wg_per_cu * 2 + k.xcd;  // workgroups per CU (0: one per tile), times 2, plus the XCD flag
+    return HD_OK;
+}
+
+int hd_ens_set_launch(hd_ens* e, int32_t choice) {
+    if (!e) return fail(HD_ERR_INVALID, "NULL ensemble");
+    return set_launch(e, choice);
+}
+
+int hd_ens_get_launch(const hd_ens* e) { return e ? e->choice : -1; }
+
+int hd_ens_tune(hd_ens* e, double* q, double* scratch, const double* dt_dev, int32_t reps, void* stream,
+                int32_t* choice, double* us_per_stage) {
+    if (!e || !q || !scratch || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q == scratch) return fail(HD_ERR_INVALID, "q and scratch must differ");
+    if (reps < 1) reps = 1;
+    Ops o;
+    if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ens_noise(e, dt_dev, s);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(HD_ERR_HIP, "hipEventCreate failed");
+    }
+    int best = -1;
+    float best_ms = 0.f;
+    for (int c = 0; c < kNumCandidates && rc == HD_OK; c++) {
+        if ((rc = set_launch(e, c))) break;
+        // one untimed stage, then reps timed stages of the same form as a step's first stage
+        StageParams sp = ens_params(e, dt_dev, 0, 0);
+        o.ens_stage(e, q, q, scratch, sp, s);
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(HD_ERR_HIP, "hipEventRecord failed"); break; }
+        for (int n = 0; n < reps; n++) o.ens_stage(e, q, q, scratch, sp, s);
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(HD_ERR_HIP, "timing the launch shapes failed");
+            break;
+        }
+        if (us_per_stage) us_per_stage[c] = 1000.0 * ms / reps;
+        if (best < 0 || ms < best_ms) { best = c; best_ms = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    // the tuning stages read the state only; clear any flag they raised
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemset(e->err, 0, sizeof(int)));
+    HIPCHK(hipMemset(e->bad, 0xff, sizeof(unsigned long long)));
+    if ((rc = set_launch(e, best))) return rc;
+    if (choice) *choice = best;
+    return HD_OK;
+}
+
+}  // extern "C"

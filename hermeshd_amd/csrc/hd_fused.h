@@ -407,16 +407,13 @@ __global__ __launch_bounds__(256, (NB == 4 ? 4 : 2)) void k_limit(double* __rest
 // ------------------------------------------------------------ kernel -------
 __device__ __forceinline__ void fence() { asm volatile("" ::: "memory"); }
 
+// One cell of one stage; `bad_id` is the index reported when mode 1 of the
+// output is NaN.
 template <int NB, int QP, int NQ, int RM, bool NZ>
-__global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
-                                               double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.N) return;
-    CellPos cp;
-    cp.c = c;
-    cp.i = (int)(c % g.nx);
-    cp.j = (int)((c / g.nx) % g.ny);
-    cp.k = (int)(c / ((long)g.nx * g.ny));
+__device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const double* __restrict__ qn,
+                                           double* __restrict__ qout, const Grid& g, const Phys& ph,
+                                           const StageParams& sp, const CellPos& cp, long bad_id) {
+    const long c = cp.c;
     double m[NQ][NB];
     load_modes<NB, NQ>(qin, g, c, m);
     double rhs[NQ][NB];
@@ -492,8 +489,21 @@ __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, c
         for (int r = 0; r < NB; r++) qout[((long)r * NQ + f) * g.N + c] = rhs[f][r];
     if (bad) {
         atomicOr(sp.err, 1);
-        atomicMin(sp.bad, (unsigned long long)c);
+        atomicMin(sp.bad, (unsigned long long)bad_id);
     }
+}
+
+template <int NB, int QP, int NQ, int RM, bool NZ>
+__global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
+                                               double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    CellPos cp;
+    cp.c = c;
+    cp.i = (int)(c % g.nx);
+    cp.j = (int)((c / g.nx) % g.ny);
+    cp.k = (int)(c / ((long)g.nx * g.ny));
+    stage_cell<NB, QP, NQ, RM, NZ>(qin, qn, qout, g, ph, sp, cp, c);
 }
 
 }  // namespace fz

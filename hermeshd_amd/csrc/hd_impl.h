@@ -2,7 +2,6 @@
 // configuration; included by the instantiation units only.
 #pragma once
 #include "hd_ctx.h"
-#include "hd_fused.h"
 
 namespace hd {
 
@@ -36,7 +35,23 @@ struct Impl {
         hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, false>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, qin, qn,
                            qout, c->grid, c->ph, sp);
     }
-    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused}; }
+    static void ens_limit(hd_ens* e, double* q, hipStream_t s) {
+        hipLaunchKernelGGL((ens::k_ens_limit<NB, QP>), dim3(e->grid), dim3(e->bs), 0, s, q, e->dreal, e->dwork,
+                           e->launch, e->lp);
+    }
+    static void ens_stage(hd_ens* e, const double* qin, const double* qn, double* qout, const StageParams& sp,
+                          hipStream_t s) {
+        if constexpr (NQ >= 11) {
+            if (e->base.noise) {
+                hipLaunchKernelGGL((ens::k_ens_stage<NB, QP, NQ, RM, true>), dim3(e->grid), dim3(e->bs), 0, s, qin,
+                                   qn, qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((ens::k_ens_stage<NB, QP, NQ, RM, false>), dim3(e->grid), dim3(e->bs), 0, s, qin, qn,
+                           qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise);
+    }
+    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage}; }
 };
 
 template <int NB, int QP, int NQ>

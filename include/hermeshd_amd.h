@@ -115,6 +115,62 @@ int hd_set_step(hd_ctx *ctx, uint64_t step, void *stream);
    this point (src/integrator.f90:234-250); here the caller decides. */
 int hd_check(hd_ctx *ctx, void *stream, int64_t *bad_cell);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent realizations of one configuration on boxes of
+ * different sizes, advanced together.  The reference runs one realization
+ * per process (the time loop of src/main.f90:22-49 around step,
+ * src/hermeshd.f90:53-62); each realization here computes exactly what one
+ * context with the same configuration and realization id computes.
+ * The states of all realizations are packed back to back in one device
+ * array, each in the layout above; hd_ens_offsets gives where each starts.
+ * `base` gives the physics, basis, boundaries and noise; its nx,ny,nz and
+ * lx,ly,lz only fix the cell size (lx/nx, ...), shared by every realization.
+ * dims holds nreal (nx,ny,nz) triples.  realization_ids (NULL: base
+ * realization + index) key each realization's noise.
+ * ------------------------------------------------------------------------ */
+typedef struct hd_ens hd_ens;
+
+int hd_ens_create(const hd_config *base, int32_t nreal, const int32_t *dims,
+                  const uint32_t *realization_ids, hd_ens **out);
+int hd_ens_destroy(hd_ens *ens);
+/* doubles in one packed state array */
+int64_t hd_ens_state_size(const hd_ens *ens);
+/* nreal start offsets (in doubles) of the realizations in a packed array */
+int hd_ens_offsets(const hd_ens *ens, int64_t *offsets);
+int64_t hd_ens_device_bytes(const hd_ens *ens);
+
+/* hd_limit, hd_rhs and hd_update over every realization; one launch per
+   stage for all of them, one noise launch per step. */
+int hd_ens_limit(hd_ens *ens, double *q, void *stream);
+int hd_ens_rhs(hd_ens *ens, const double *q, double *rhs, void *stream);
+int hd_ens_update(hd_ens *ens, double *q_io, double *q1, double *q2, const double *dt_dev,
+                  void *stream);
+
+/* nsteps steps of hd_ens_update, replayed from a HIP graph of one step that
+   is captured on the first call (and again when an argument or the launch
+   shape changes).  *dt_dev may change between calls. */
+int hd_ens_run(hd_ens *ens, double *q_io, double *q1, double *q2, const double *dt_dev,
+               int64_t nsteps, void *stream);
+
+int hd_ens_set_step(hd_ens *ens, uint64_t step, void *stream);
+/* like hd_check; reports the realization and its cell */
+int hd_ens_check(hd_ens *ens, void *stream, int64_t *bad_realization, int64_t *bad_cell);
+
+/* Launch shapes of the stage kernel: tiles of consecutive cells or bricks,
+   one workgroup per tile or a persistent grid walking the (realization,
+   tile) list.  info: block size, brick bx,by,bz, linear flag, and
+   2*(workgroups per CU, 0 = one per tile) + (XCD-contiguous chunks).  */
+int32_t hd_ens_launch_count(void);
+int hd_ens_launch_info(int32_t choice, int32_t info[6]);
+int hd_ens_set_launch(hd_ens *ens, int32_t choice);
+int hd_ens_get_launch(const hd_ens *ens);
+/* Autotuner: times reps stages of every launch shape on q (read only; the
+   output goes to scratch), keeps the fastest and returns it in *choice;
+   us_per_stage (NULL or hd_ens_launch_count() doubles) receives the times.
+   Synchronises the device. */
+int hd_ens_tune(hd_ens *ens, double *q, double *scratch, const double *dt_dev, int32_t reps,
+                void *stream, int32_t *choice, double *us_per_stage);
+
 /* Stage timing for measurement: when enabled, every stage launched by
    hd_euler_step/hd_update/hd_step is bracketed by HIP events recorded on the
    caller's stream.  hd_timing_read synchronises, returns the summed stage time
