@@ -142,6 +142,92 @@ def measure_traffic(workload, kernel_key):
                            "copy_factor_fetch": round(ff, 4), "copy_factor_write": round(fw, 4)}
 
 
+def adjoint_leg(nsteps=400, checkpoint_every=20):
+    """Cost of the gradient of the relaxation channel (64x128, nbasis 4,
+    nQ 11, ivis 1, HLLC, no-slip walls, 400 steps, dt 0.05, checkpoints
+    every 20 steps) with respect to the initial velocity and the relaxation
+    rate: forward run vs backward (recompute + stage transposes)."""
+    import torch
+    from hermeshd_amd import ics
+    from hermeshd_amd.adjoint import AdjointRunner, channel_state, run_with_grad, wall_shear
+    from hermeshd_amd.config import poiseuille_2d
+    cfg = poiseuille_2d()
+    dev = torch.device("cuda:0")
+    u0 = torch.from_numpy(0.1 * ics.gaussian_field((1, cfg.ny, cfg.nx), 8.0, 1234)).to(dev)
+    runner = AdjointRunner(cfg, dt=0.05, nsteps=nsteps, checkpoint_every=checkpoint_every)
+
+    def once():
+        u = u0.clone().requires_grad_(True)
+        c = torch.tensor(float(cfg.coll), dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = wall_shear(run_with_grad(runner, channel_state(cfg, u), c))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gu, gc = torch.autograd.grad(J, (u, c))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(J), float(gc)
+    once()
+    tf, tb, J, gc = once()
+    return {"workload": f"relaxation channel 64x128, nbasis 4, nQ 11, ivis 1, HLLC, no-slip walls, "
+                        f"{nsteps} SSP-RK2 steps, dt 0.05, checkpoints every {checkpoint_every}",
+            "forward_ms": round(tf * 1e3, 2), "backward_ms": round(tb * 1e3, 2), "ratio": round(tb / tf, 3),
+            "J_wall_shear": J, "dJ_dcoll": gc}
+
+
+def ensemble_leg(steps=20):
+    """Batched ensemble (96x16^3 + 96x24^3 + 64x32^3 periodic boxes, nbasis 4,
+    nQ 11, stochastic stress, HLLC, SSP-RK2, dt 0.1, graph-replayed steps,
+    tuned launch shape) against the same realizations stepped one at a time
+    through single contexts."""
+    import torch
+    from hermeshd_amd import ics
+    from hermeshd_amd.config import fluct_ensemble
+    from hermeshd_amd.ensemble import Ensemble
+    from hermeshd_amd.solver import Solver
+    base, dims = fluct_ensemble()
+    ens = Ensemble(base, dims)
+
+    def unif(r):
+        nx, ny, nz = dims[r]
+        return ics.uniform(base.replace(nx=nx, ny=ny, nz=nz, lx=float(nx), ly=float(ny), lz=float(nz)),
+                           rho=1.0, T=1.0)
+    states = [unif(r) for r in range(ens.nreal)]
+    q = ens.pack(states)
+    q1 = ens.zeros()
+    choice, _ = ens.tune(q, ens.zeros(), 0.1, reps=3)
+    ens.run(q, q1, None, 0.1, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(q, q1, None, 0.1, steps)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    ens.check()
+    sols, qs, q1s = [], [], []
+    for r in range(ens.nreal):
+        nx, ny, nz = dims[r]
+        cfg = base.replace(nx=nx, ny=ny, nz=nz, lx=float(nx), ly=float(ny), lz=float(nz), realization=r)
+        sols.append(Solver(cfg))
+        qs.append(torch.from_numpy(states[r]).cuda())
+        q1s.append(torch.zeros_like(qs[-1]))
+    for s_, a, b in zip(sols, qs, q1s):
+        s_.update(a, b, None, 0.1)
+    torch.cuda.synchronize()
+    n1 = max(2, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(n1):
+        for s_, a, b in zip(sols, qs, q1s):
+            s_.update(a, b, None, 0.1)
+    torch.cuda.synchronize()
+    t1 = (time.perf_counter() - t0) / n1
+    dofs = ens.cells * base.nQ * base.nbasis
+    return {"workload": "256 periodic realizations (96x16^3, 96x24^3, 64x32^3), nbasis 4, nQ 11, stochastic "
+                        "stress, HLLC, SSP-RK2, dt 0.1",
+            "batched_ms_per_step": round(tb * 1e3, 3), "one_at_a_time_ms_per_step": round(t1 * 1e3, 3),
+            "speedup": round(t1 / tb, 2), "batched_dof_updates_per_s": dofs * 2 / tb, "launch_shape": choice}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +236,8 @@ def main():
     ap.add_argument("--workload", default="vortex", choices=["vortex", "sod", "channel"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -259,6 +347,14 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline (oracle, 1 thread)...")
         out["cpu_baseline"] = cpu_baseline(cfg, make_ic, stages)
+    if not args.no_adjoint:
+        log("bench: adjoint leg (relaxation channel, 400 steps)...")
+        out["adjoint"] = adjoint_leg()
+        log(f"bench: adjoint {out['adjoint']}")
+    if not args.no_ensemble:
+        log("bench: ensemble leg (256 realizations)...")
+        out["ensemble"] = ensemble_leg()
+        log(f"bench: ensemble {out['ensemble']}")
     print(json.dumps(out), flush=True)
 
 

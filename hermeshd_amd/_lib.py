@@ -17,7 +17,7 @@ HD_ERR_UNSUPPORTED = -4
 # every extern "C" symbol declared in include/hermeshd_amd.h
 EXPORTED = [
     "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_update",
-    "hd_step", "hd_set_step", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
+    "hd_step", "hd_stage", "hd_rhs_vjp", "hd_stage_vjp", "hd_set_step", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
     "hd_last_error",
     "hd_ens_create", "hd_ens_destroy", "hd_ens_state_size", "hd_ens_offsets", "hd_ens_device_bytes",
     "hd_ens_limit", "hd_ens_rhs", "hd_ens_update", "hd_ens_run", "hd_ens_set_step", "hd_ens_check",
@@ -38,6 +38,14 @@ class HDConfigC(C.Structure):
         ("rh_floor", C.c_double), ("T_floor", C.c_double), ("coll", C.c_double), ("colvis", C.c_double),
         ("cflm", C.c_double), ("body_force", C.c_double * 3), ("eta_sd", C.c_double), ("zeta_sd", C.c_double),
         ("seed", C.c_uint64), ("realization", C.c_uint32), ("reserved1", C.c_uint32),
+    ]
+
+
+class StageVjpArgs(C.Structure):
+    _fields_ = [
+        ("a", C.c_double), ("b", C.c_double), ("acc0", C.c_void_p), ("acc1", C.c_void_p),
+        ("w0", C.c_double), ("w1", C.c_double), ("q_raw", C.c_void_p), ("out", C.c_void_p), ("qn", C.c_void_p),
+        ("dcoll_dev", C.c_void_p),
     ]
 
 
@@ -74,6 +82,9 @@ def lib() -> C.CDLL:
     L.hd_update.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hd_step.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.hd_set_step.argtypes = [vp, C.c_uint64, vp]
+    L.hd_stage.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
+    L.hd_rhs_vjp.argtypes = [vp, vp, vp, vp, vp]
+    L.hd_stage_vjp.argtypes = [vp, vp, vp, vp, vp, C.POINTER(StageVjpArgs), vp]
     L.hd_check.argtypes = [vp, vp, C.POINTER(C.c_int64)]
     L.hd_timing_enable.argtypes = [vp, C.c_int]
     L.hd_timing_read.argtypes = [vp, dp, C.POINTER(C.c_int64)]

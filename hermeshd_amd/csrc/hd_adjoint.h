@@ -1,0 +1,657 @@
+// hd_adjoint.h -- hand-written reverse mode (vector-Jacobian products) of the
+// RK stage, for gradients of a run with respect to its initial state and the
+// relaxation rate.
+//
+// For a stage  out = a*Q^n + b*v,  v = F_ivis(m + dt*R(m)),  R = -glflux + source
+// (src/integrator.f90:62-252; F_ivis scales the stress fields by
+// 1/(1 + dt*coll) when ivis = 1), the kernel k_stage_vjp computes, for one
+// cell c,
+//     lam_in[c] = sum_k w_k acc_k[c] + L'(raw)^T ( lp[c] + dt * sum_j (dR_j/dm_c)^T lp[j] )
+// with lp = F_ivis'^T (b*lam_out): the transpose of the stage map applied to
+// the cotangent of its output.  (dR/dm)^T is gathered per cell: the cell's
+// own volume term, and for each of its six faces the face flux's Jacobian
+// with respect to this cell's trace, applied to the cotangent that the face
+// flux receives from both cells that use it (volume: innerintegral2,
+// src/flux.f90:751-928; faces: calc_flux_x/y/z and flux_hllc,
+// src/flux.f90:197-647; lift: glflux, src/flux.f90:932-1022; sources:
+// src/sources.f90:12-84).  L' is the limiter (src/flux.f90:1077-1164) with
+// its active set and factors frozen at the forward values: the factors that
+// scaled the modes scale the cotangent.
+//
+// The point functions are differentiated by hand: pr::flux (physical flux
+// from primitives) and pr::hllc (Batten HLLC with Davis speeds), each
+// recomputing its forward intermediates and sweeping them backwards.  Branch
+// decisions (wave-speed selection, min/max, floors, boundary-ghost choices)
+// are those of the forward evaluation at the same state.  The stochastic
+// stress and the Roe solver have no adjoint here.
+#pragma once
+#include "hd_fused.h"
+
+namespace hd {
+namespace adj {
+
+// ---------------------------------------------------------------- flux ----
+// reverse of pr::prim + pr::flux<NQ,DIR>: bF (cotangent of F) -> bQ (+=)
+template <int NQ, int DIR>
+__device__ __forceinline__ void flux_vjp(const Phys& ph, const double* Q, const pr::Prim& s, const double* bF,
+                                         double* bQ) {
+    constexpr int ND = MX + DIR;
+    const double un = s.u[DIR];
+    double Pn[3] = {0., 0., 0.};
+    int pidx[3] = {0, 0, 0};
+    if constexpr (NQ >= 11) {
+        if (DIR == 0) { pidx[0] = PXX; pidx[1] = PXY; pidx[2] = PXZ; }
+        if (DIR == 1) { pidx[0] = PXY; pidx[1] = PYY; pidx[2] = PYZ; }
+        if (DIR == 2) { pidx[0] = PXZ; pidx[1] = PYZ; pidx[2] = PZZ; }
+        Pn[0] = Q[pidx[0]]; Pn[1] = Q[pidx[1]]; Pn[2] = Q[pidx[2]];
+    }
+    const double P = s.pf;
+    double bun = bF[MX] * Q[MX] + bF[MY] * Q[MY] + bF[MZ] * Q[MZ] + bF[EN] * (Q[EN] + P);
+    bQ[MX] += bF[MX] * un;
+    bQ[MY] += bF[MY] * un;
+    bQ[MZ] += bF[MZ] * un;
+    bQ[EN] += bF[EN] * un;
+    bQ[ND] += bF[RH];
+    const double bP = bF[ND] + bF[EN] * un;
+    double bu[3] = {bF[EN] * Pn[0], bF[EN] * Pn[1], bF[EN] * Pn[2]};
+    if constexpr (NQ >= 11) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) bQ[pidx[a]] += bF[MX + a] + bF[EN] * s.u[a];
+        bun += (DIR == 0 ? ph.c4d3cv : -ph.c2d3cv) * bF[PXX] + (DIR == 1 ? ph.c4d3cv : -ph.c2d3cv) * bF[PYY] +
+               (DIR == 2 ? ph.c4d3cv : -ph.c2d3cv) * bF[PZZ];
+        if (DIR == 0) { bu[1] += ph.colvis * bF[PXY]; bu[2] += ph.colvis * bF[PXZ]; }
+        if (DIR == 1) { bu[0] += ph.colvis * bF[PXY]; bu[2] += ph.colvis * bF[PYZ]; }
+        if (DIR == 2) { bu[0] += ph.colvis * bF[PXZ]; bu[1] += ph.colvis * bF[PYZ]; }
+    }
+    bu[DIR] += bun;
+    // primitives: u_a = m_a/rho, P = gm1*(E - 0.5/rho*|m|^2) unless floored
+    double bri = 0.;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        bQ[MX + a] += bu[a] * s.ri;
+        bri += bu[a] * Q[MX + a];
+    }
+    const double smsq = Q[MX] * Q[MX] + Q[MY] * Q[MY] + Q[MZ] * Q[MZ];
+    const double Praw = ph.gm1 * (Q[EN] - 0.5 * s.ri * smsq);
+    if (!(Praw < ph.P_floor)) {
+        bQ[EN] += ph.gm1 * bP;
+        bri += -0.5 * ph.gm1 * smsq * bP;
+#pragma unroll
+        for (int a = 0; a < 3; a++) bQ[MX + a] += -ph.gm1 * s.ri * Q[MX + a] * bP;
+    }
+    bQ[RH] += -s.ri * s.ri * bri;
+}
+
+// ---------------------------------------------------------------- HLLC ----
+// reverse of pr::hllc<DIR> for the Euler fields: mu (cotangent of out[0..4])
+// -> bQL, bQR (+=, Euler fields), bFL, bFR (+=, Euler fields).  Returns false
+// when no branch applied (out kept the central flux).
+template <int DIR>
+__device__ __forceinline__ bool hllc_vjp(const Phys& ph, const double* QL, const pr::Prim& L, const double* QR,
+                                         const pr::Prim& R, const double* mu, double* bQL, double* bQR, double* bFL,
+                                         double* bFR) {
+    constexpr int IP = MX + DIR, I1 = MX + (DIR + 1) % 3, I2 = MX + (DIR + 2) % 3;
+    // ---- forward, as pr::hllc
+    const double rL = QL[RH], rR = QR[RH];
+    const double uL = L.u[DIR], uR = R.u[DIR];
+    const double v2L = L.u[0] * L.u[0] + L.u[1] * L.u[1] + L.u[2] * L.u[2];
+    const double v2R = R.u[0] * R.u[0] + R.u[1] * R.u[1] + R.u[2] * R.u[2];
+    const double pL = ph.gm1 * (QL[EN] - 0.5 * rL * v2L);
+    const double pR = ph.gm1 * (QR[EN] - 0.5 * rR * v2R);
+    const double cL = sqrt(ph.gamma * pL * L.ri), cR = sqrt(ph.gamma * pR * R.ri);
+    const bool sl_left = (uL - cL) <= (uR - cR);   // fmin picks uL - cL
+    const bool sr_right = (uR + cR) >= (uL + cL);  // fmax picks uR + cR
+    const double SL = fmin(uL - cL, uR - cR);
+    const double SR = fmax(uR + cR, uL + cL);
+    const double aR = rR * (SR - uR), aL = rL * (SL - uL);
+    double sm_den = aR - aL;
+    const bool den_floor = (sm_den == 0.0);
+    if (den_floor) sm_den = ph.rh_floor;
+    const double SM = pr::divide(pr::contact_num(aR, uR, aL, uL, pL, pR), sm_den);
+    int sel = -1;
+    if (SL > 0.0) sel = 0;
+    if (SL <= 0.0 && 0.0 < SM) sel = 1;
+    if (SM <= 0.0 && 0.0 <= SR) sel = 2;
+    if (SR < 0.0) sel = 3;
+    if (sel < 0) return false;
+    if (sel == 0) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) bFL[f] += mu[f];
+        return true;
+    }
+    if (sel == 3) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) bFR[f] += mu[f];
+        return true;
+    }
+    const bool left = (sel == 1);
+    const double* Qk = left ? QL : QR;
+    const double Sk = left ? SL : SR;
+    const double uk = left ? uL : uR, pk = left ? pL : pR;
+    const double pstar = rL * (uL - SL) * (uL - SM) + pL;
+    double den = Sk - SM;
+    const bool kden_floor = (den == 0.0);
+    if (kden_floor) den = ph.rh_floor;
+    const double rid = pr::rcp(den);
+    const double sq = Sk - uk;
+    const double t1 = sq * Qk[IP] + pstar - pk;
+    const double t4 = sq * Qk[EN] - pk * uk + pstar * SM;
+    const double s[5] = {Qk[RH] * sq * rid, t1 * rid, sq * Qk[I1] * rid, sq * Qk[I2] * rid, t4 * rid};
+    const int fld[5] = {RH, IP, I1, I2, EN};
+    // ---- reverse: out[f] = Fk[f] + Sk*(s_f - Qk[f])
+    double bQk[5] = {0., 0., 0., 0., 0.};  // indexed like fld
+    double bSk = 0., bs[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        const double m = mu[fld[e]];
+        bSk += m * (s[e] - Qk[fld[e]]);
+        bs[e] = m * Sk;
+        bQk[e] -= m * Sk;
+    }
+    double* bFk = left ? bFL : bFR;
+#pragma unroll
+    for (int f = 0; f < 5; f++) bFk[f] += mu[f];
+    double bsq = 0., brid = 0., bpstar = 0., bpk = 0., buk = 0., bSM = 0.;
+    // s0 = Qk[RH]*sq*rid
+    bQk[0] += bs[0] * sq * rid; bsq += bs[0] * Qk[RH] * rid; brid += bs[0] * Qk[RH] * sq;
+    // s1 = t1*rid
+    bsq += bs[1] * Qk[IP] * rid; bQk[1] += bs[1] * sq * rid; bpstar += bs[1] * rid; bpk -= bs[1] * rid;
+    brid += bs[1] * t1;
+    // s2, s3
+    bsq += bs[2] * Qk[I1] * rid; bQk[2] += bs[2] * sq * rid; brid += bs[2] * sq * Qk[I1];
+    bsq += bs[3] * Qk[I2] * rid; bQk[3] += bs[3] * sq * rid; brid += bs[3] * sq * Qk[I2];
+    // s4 = t4*rid
+    bsq += bs[4] * Qk[EN] * rid; bQk[4] += bs[4] * sq * rid; bpk -= bs[4] * uk * rid; buk -= bs[4] * pk * rid;
+    bpstar += bs[4] * SM * rid; bSM += bs[4] * pstar * rid; brid += bs[4] * t4;
+    // rid = 1/den, den = Sk - SM
+    if (!kden_floor) {
+        const double bden = -brid * rid * rid;
+        bSk += bden;
+        bSM -= bden;
+    }
+    // sq = Sk - uk
+    bSk += bsq;
+    buk -= bsq;
+    // pstar = rL*(uL-SL)*(uL-SM) + pL
+    double brL = bpstar * (uL - SL) * (uL - SM);
+    double buL = bpstar * rL * ((uL - SM) + (uL - SL));
+    double bSL = -bpstar * rL * (uL - SM);
+    bSM -= bpstar * rL * (uL - SL);
+    double bpL = bpstar, bpR = 0., brR = 0., buR = 0., bSR = 0.;
+    if (left) { bpL += bpk; buL += buk; bSL += bSk; } else { bpR += bpk; buR += buk; bSR += bSk; }
+    // SM = sm_num/sm_den
+    const double bnum = bSM / sm_den;
+    const double bdenm = den_floor ? 0. : -bSM * SM / sm_den;
+    double baR = bnum * uR + bdenm, baL = -bnum * uL - bdenm;
+    buR += bnum * aR;
+    buL -= bnum * aL;
+    bpL += bnum;
+    bpR -= bnum;
+    // aR = rR*(SR-uR), aL = rL*(SL-uL)
+    brR += baR * (SR - uR); bSR += baR * rR; buR -= baR * rR;
+    brL += baL * (SL - uL); bSL += baL * rL; buL -= baL * rL;
+    // SL = min(uL-cL, uR-cR), SR = max(uR+cR, uL+cL)
+    double bcL = 0., bcR = 0.;
+    if (sl_left) { buL += bSL; bcL -= bSL; } else { buR += bSL; bcR -= bSL; }
+    if (sr_right) { buR += bSR; bcR += bSR; } else { buL += bSR; bcL += bSR; }
+    // c = sqrt(gamma*p*ri)
+    double briL = 0., briR = 0.;
+    if (cL > 0.) { bpL += bcL * ph.gamma * L.ri / (2. * cL); briL += bcL * ph.gamma * pL / (2. * cL); }
+    if (cR > 0.) { bpR += bcR * ph.gamma * R.ri / (2. * cR); briR += bcR * ph.gamma * pR / (2. * cR); }
+    // scatter the star-state field cotangents onto side k
+    double* bQkf = left ? bQL : bQR;
+    bQkf[RH] += bQk[0]; bQkf[IP] += bQk[1]; bQkf[I1] += bQk[2]; bQkf[I2] += bQk[3]; bQkf[EN] += bQk[4];
+    // p = gm1*(E - 0.5*rho*|u|^2), u_a = m_a*ri, ri = 1/rho; u_dir also enters directly
+    {
+        double bu[3] = {0., 0., 0.};
+        bu[DIR] += buL;
+        bQL[EN] += ph.gm1 * bpL;
+        brL += -0.5 * ph.gm1 * v2L * bpL;
+#pragma unroll
+        for (int a = 0; a < 3; a++) bu[a] += -ph.gm1 * rL * L.u[a] * bpL;
+#pragma unroll
+        for (int a = 0; a < 3; a++) { bQL[MX + a] += bu[a] * L.ri; briL += bu[a] * QL[MX + a]; }
+        bQL[RH] += brL - L.ri * L.ri * briL;
+    }
+    {
+        double bu[3] = {0., 0., 0.};
+        bu[DIR] += buR;
+        bQR[EN] += ph.gm1 * bpR;
+        brR += -0.5 * ph.gm1 * v2R * bpR;
+#pragma unroll
+        for (int a = 0; a < 3; a++) bu[a] += -ph.gm1 * rR * R.u[a] * bpR;
+#pragma unroll
+        for (int a = 0; a < 3; a++) { bQR[MX + a] += bu[a] * R.ri; briR += bu[a] * QR[MX + a]; }
+        bQR[RH] += brR - R.ri * R.ri * briR;
+    }
+    return true;
+}
+
+// face flux reverse: mu (cotangent of the face flux, all fields) -> bQL, bQR (+=)
+template <int NQ, int RM, int DIR>
+__device__ __forceinline__ void face_vjp(const Phys& ph, const double* QL, const double* QR, const double* mu,
+                                         double* bQL, double* bQR) {
+    const pr::Prim sL = pr::prim<NQ>(ph, QL), sR = pr::prim<NQ>(ph, QR);
+    double bFL[NQ], bFR[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) { bFL[f] = 0.5 * mu[f]; bFR[f] = 0.5 * mu[f]; }
+    if constexpr (RM == RIEMANN_HLLC) {
+        double zL[5] = {0., 0., 0., 0., 0.}, zR[5] = {0., 0., 0., 0., 0.};
+        if (hllc_vjp<DIR>(ph, QL, sL, QR, sR, mu, bQL, bQR, zL, zR)) {
+            // HLLC replaced the central flux of rh..en
+#pragma unroll
+            for (int f = 0; f < 5; f++) { bFL[f] = zL[f]; bFR[f] = zR[f]; }
+        }
+    }
+    flux_vjp<NQ, DIR>(ph, QL, sL, bFL, bQL);
+    flux_vjp<NQ, DIR>(ph, QR, sR, bFR, bQR);
+}
+
+// ------------------------------------------------------------ limiter ----
+// The limiter's factors for one cell (the same decisions and values as
+// fz::limit_cell): modes >= 2 of rh are scaled by th_rho and then, with the
+// other Euler fields, by th_p; `zero` zeroes modes >= 2 of rh..en; `floor`
+// also replaces the density average by rh_floor.
+struct LimFactors {
+    bool floor, zero;
+    double th_rho, th_p;
+};
+
+template <int NB, int QP, int MQ>
+__device__ __forceinline__ LimFactors limit_factors(const double (&m0)[MQ][NB], const LimParams& lp) {
+    LimFactors F{false, false, 1.0, 1.0};
+    if (m0[RH][0] < lp.rh_floor) {
+        F.floor = true;
+        F.zero = true;
+        return F;
+    }
+    const auto& T = fz::face_tab<NB>();
+    double rho[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) rho[r] = m0[RH][r];
+    double rmin = 0.;
+#pragma unroll 1
+    for (int p = 0; p < 6 * QP * QP; p++) {
+        const double* b = T.bf[p / (2 * QP * QP)][(p / (QP * QP)) % 2][p % (QP * QP)];
+        double s = 0.;
+#pragma unroll
+        for (int r = 0; r < NB; r++) s += b[r] * rho[r];
+        rmin = (p == 0) ? s : (s < rmin ? s : rmin);
+    }
+    if (rmin < lp.rh_floor) {
+        double theta = (lp.rh_floor - rho[0]) / (rmin - rho[0]);
+        if (theta > 1.) theta = 1.;
+        if (theta < 0.) theta = 0.;
+        F.th_rho = fabs(theta);
+#pragma unroll
+        for (int r = 1; r < NB; r++) rho[r] = F.th_rho * rho[r];
+    }
+    const double Pave =
+        lp.gm1 * (m0[EN][0] - 0.5 * (m0[MX][0] * m0[MX][0] + m0[MY][0] * m0[MY][0] + m0[MZ][0] * m0[MZ][0]) / rho[0]);
+    if (Pave < lp.epsP) {
+        F.zero = true;
+        return F;
+    }
+    double pmin = 0.;
+#pragma unroll 1
+    for (int p = 0; p < 6 * QP * QP; p++) {
+        const double* b = T.bf[p / (2 * QP * QP)][(p / (QP * QP)) % 2][p % (QP * QP)];
+        double e[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += b[r] * (f == RH ? rho[r] : m0[f][r]);
+            e[f] = s;
+        }
+        const double dni = pr::rcp(e[RH]);
+        const double P = lp.gm1 * (e[EN] - 0.5 * dni * (e[MX] * e[MX] + e[MY] * e[MY] + e[MZ] * e[MZ]));
+        pmin = (p == 0) ? P : (P < pmin ? P : pmin);
+    }
+    if (pmin < lp.epsP && Pave != pmin) {
+        double theta = (Pave - lp.epsP) / (Pave - pmin);
+        if (theta > 1.) theta = 1.;
+        if (theta < 0.) theta = 0.;
+        F.th_p = theta;
+    }
+    return F;
+}
+
+// cotangent through the frozen limiter, in place
+template <int NB, int MQ>
+__device__ __forceinline__ void limit_vjp(double (&lam)[MQ][NB], const LimFactors& F) {
+    if (F.zero) {
+#pragma unroll
+        for (int r = 1; r < NB; r++)
+#pragma unroll
+            for (int f = 0; f < 5; f++) lam[f][r] = 0.0;
+        if (F.floor) lam[RH][0] = 0.0;
+        return;
+    }
+#pragma unroll
+    for (int r = 1; r < NB; r++) {
+        lam[RH][r] *= F.th_rho * F.th_p;
+#pragma unroll
+        for (int f = 1; f < 5; f++) lam[f][r] *= F.th_p;
+    }
+}
+
+// ------------------------------------------------------------- stage -----
+struct AdjParams {
+    const double* dt;          // device time step
+    double coll;
+    int ivis;
+    double a, b;               // out = a*Q^n + b*v
+    double bf[3];              // body force
+    const double* acc[2];      // lam_in += w[k]*acc[k] (nullptr: none)
+    double w[2];
+    const double* q_raw;       // stage input before the limiter (nullptr: no limiter)
+    LimParams lp;
+    const double* out;         // forward stage output and Q^n, for d/dcoll (ivis 1)
+    const double* qn;
+    double* partial;           // per-block partial sums of d(lam.out)/dcoll
+    int rhs_only;              // 1: lam_in = (dR/dm)^T lam (no stage map, no limiter, no sums)
+};
+
+template <int NB, int NQ>
+__device__ __forceinline__ void load_lp(const double* __restrict__ lam, const Grid& g, long c, double scale,
+                                        double faci, double (&l)[NQ][NB]) {
+#pragma unroll
+    for (int r = 0; r < NB; r++)
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            double v = scale * lam[((long)r * NQ + f) * g.N + c];
+            if (f >= PXX) v *= faci;
+            l[f][r] = v;
+        }
+}
+
+// one face of the cell: adds this cell's share of (dR/dm)^T lp to rt
+template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
+__device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, const double* __restrict__ lam,
+                                              const Grid& g, const fz::CellPos& cp, const double (&m)[NQ][NB],
+                                              const double (&lo)[NQ][NB], double (&rt)[NQ][NB], const Phys& ph,
+                                              double dxi_d, double scale, double faci) {
+    constexpr int NF = QP * QP;
+    const auto& T = fz::face_tab<NB>();
+    const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int a = DIR == 0 ? cp.i : DIR == 1 ? cp.j : cp.k;
+    const bool at_edge = SIDE ? (a == n_ax - 1) : (a == 0);
+    const int bc = g.bc[2 * DIR + SIDE];
+    const bool ext = at_edge && bc != BC_PERIODIC;
+    double nm[NQ][NB], nl[NQ][NB];
+    if (!ext) {
+        int an = SIDE ? a + 1 : a - 1;
+        if (an < 0) an += n_ax;
+        if (an >= n_ax) an -= n_ax;
+        const long cn = DIR == 0 ? (cp.c - cp.i + an)
+                      : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
+                                 : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
+        fz::load_modes<NB, NQ>(q, g, cn, nm);
+        load_lp<NB, NQ>(lam, g, cn, scale, faci, nl);
+    }
+    bool zero_normal = false;
+    if (ext && bc == BC_OUTFLOW && !(DIR == 0 && SIDE == 1)) {
+        double v = 0.;
+        for (int p = 0; p < NF; p++) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) s += T.bf[DIR][SIDE][p][r] * m[MX + DIR][r];
+            v = (p == 0) ? s : (SIDE ? fmin(v, s) : fmax(v, s));
+        }
+        zero_normal = SIDE ? (v < 0.0) : (v > 0.0);
+    }
+#pragma unroll 1
+    for (int p = 0; p < NF; p++) {
+        double own[NQ], oth[NQ], mu[NQ];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            double s = 0., t = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                s += T.bf[DIR][SIDE][p][r] * m[f][r];
+                t += T.cf[DIR][SIDE][p][r] * lo[f][r];
+            }
+            own[f] = s;
+            mu[f] = t;
+        }
+        bool flip_mx = false;
+        if (!ext) {
+#pragma unroll
+            for (int f = 0; f < NQ; f++) {
+                double s = 0., t = 0.;
+#pragma unroll
+                for (int r = 0; r < NB; r++) {
+                    s += T.bf[DIR][1 - SIDE][p][r] * nm[f][r];
+                    t += T.cf[DIR][1 - SIDE][p][r] * nl[f][r];
+                }
+                oth[f] = s;
+                mu[f] += t;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < NQ; f++) oth[f] = own[f];
+            if (bc == BC_NOSLIP) {
+                oth[MX] = oth[MY] = oth[MZ] = 0.0;
+            } else if (bc == BC_WALL) {
+                oth[MX + DIR] = 0.0;
+            } else if (bc == BC_OUTFLOW) {
+                if (DIR == 0 && SIDE == 1) {
+                    if (oth[MX] > 0.0) { oth[MX] = -own[MX]; flip_mx = true; }
+                } else if (zero_normal) {
+                    oth[MX + DIR] = 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; f++) mu[f] *= dxi_d;
+        double bL[NQ], bR[NQ];
+#pragma unroll
+        for (int f = 0; f < NQ; f++) { bL[f] = 0.; bR[f] = 0.; }
+        const double* QL = SIDE ? own : oth;
+        const double* QR = SIDE ? oth : own;
+        face_vjp<NQ, RM, DIR>(ph, QL, QR, mu, bL, bR);
+        double* bo = SIDE ? bL : bR;
+        const double* bt = SIDE ? bR : bL;
+        if (ext) {
+            // ghost = G(own): add G^T of the ghost's cotangent
+#pragma unroll
+            for (int f = 0; f < NQ; f++) {
+                double gcoef = 1.0;
+                if (bc == BC_NOSLIP && (f == MX || f == MY || f == MZ)) gcoef = 0.0;
+                if (bc == BC_WALL && f == MX + DIR) gcoef = 0.0;
+                if (bc == BC_OUTFLOW) {
+                    if (DIR == 0 && SIDE == 1) {
+                        if (f == MX && flip_mx) gcoef = -1.0;
+                    } else if (zero_normal && f == MX + DIR) {
+                        gcoef = 0.0;
+                    }
+                }
+                bo[f] += gcoef * bt[f];
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            const double v = bo[f];
+#pragma unroll
+            for (int r = 0; r < NB; r++) rt[f][r] += T.bf[DIR][SIDE][p][r] * v;
+        }
+    }
+}
+
+template <int NB, int QP, int NQ>
+__device__ __forceinline__ void volume_vjp(const double (&m)[NQ][NB], const double (&lo)[NQ][NB], double (&rt)[NQ][NB],
+                                           const Phys& ph, double dxi, double dyi, double dzi) {
+#pragma unroll
+    for (int i = 0; i < QP; i++)
+#pragma unroll
+        for (int j = 0; j < QP; j++)
+#pragma unroll
+            for (int k = 0; k < QP; k++) {
+                double Qp[NQ], mx[NQ], my[NQ], mz[NQ];
+#pragma unroll
+                for (int f = 0; f < NQ; f++) {
+                    double s = 0., ax = 0., ay = 0., az = 0.;
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        const double b = fz::phi_int<NB, QP>(r, i, j, k);
+                        if (b != 0.0) s += b * m[f][r];
+                        const double cx = fz::cvol<NB, QP>(0, r, i, j, k);
+                        const double cy = fz::cvol<NB, QP>(1, r, i, j, k);
+                        const double cz = fz::cvol<NB, QP>(2, r, i, j, k);
+                        if (r > 0 && cx != 0.0) ax += cx * lo[f][r];
+                        if (r > 0 && cy != 0.0) ay += cy * lo[f][r];
+                        if (r > 0 && cz != 0.0) az += cz * lo[f][r];
+                    }
+                    Qp[f] = s;
+                    mx[f] = dxi * ax;
+                    my[f] = dyi * ay;
+                    mz[f] = dzi * az;
+                }
+                const pr::Prim st = pr::prim<NQ>(ph, Qp);
+                double bQ[NQ];
+#pragma unroll
+                for (int f = 0; f < NQ; f++) bQ[f] = 0.;
+                flux_vjp<NQ, 0>(ph, Qp, st, mx, bQ);
+                flux_vjp<NQ, 1>(ph, Qp, st, my, bQ);
+                flux_vjp<NQ, 2>(ph, Qp, st, mz, bQ);
+#pragma unroll
+                for (int f = 0; f < NQ; f++)
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        const double b = fz::phi_int<NB, QP>(r, i, j, k);
+                        if (b != 0.0) rt[f][r] += b * bQ[f];
+                    }
+                fz::point_barrier();
+            }
+}
+
+template <int NB, int QP, int NQ, int RM>
+__global__ __launch_bounds__(256) void k_stage_vjp(const double* __restrict__ q, const double* __restrict__ lam,
+                                                   double* __restrict__ lam_in, Grid g, Phys ph, AdjParams ap) {
+    __shared__ double red[4];
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dcoll = 0.;
+    if (c < g.N) {
+        fz::CellPos cp;
+        cp.c = c;
+        cp.i = (int)(c % g.nx);
+        cp.j = (int)((c / g.nx) % g.ny);
+        cp.k = (int)(c / ((long)g.nx * g.ny));
+        const double dt = *ap.dt;
+        const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+        double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
+        fz::load_modes<NB, NQ>(q, g, c, m);
+        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) rt[f][r] = 0.;
+        volume_vjp<NB, QP, NQ>(m, lo, rt, ph, g.dxi, g.dyi, g.dzi);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        fz::fence();
+        // sources: ivis 0 explicit relaxation -coll*P; body force rho*g, m.g
+        if constexpr (NQ >= 11) {
+            if (ap.ivis == 0 && ap.coll != 0.) {
+#pragma unroll
+                for (int f = PXX; f <= PYZ; f++)
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        rt[f][r] += -ap.coll * lo[f][r];
+                        dcoll += -dt * lo[f][r] * m[f][r];
+                    }
+            }
+            if (ap.ivis == 1 && ap.coll != 0. && ap.out) {
+                // v = faci*(m + dt*R): dv/dcoll = -dt*faci*v, b*v = out - a*Q^n
+#pragma unroll
+                for (int f = PXX; f <= PYZ; f++)
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        const long o = ((long)r * NQ + f) * g.N + c;
+                        const double bv = ap.out[o] - (ap.a != 0. ? ap.a * ap.qn[o] : 0.);
+                        dcoll += -dt * faci * lam[o] * bv;
+                    }
+            }
+        }
+        if (ap.bf[0] != 0. || ap.bf[1] != 0. || ap.bf[2] != 0.) {
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                rt[RH][r] += ap.bf[0] * lo[MX][r] + ap.bf[1] * lo[MY][r] + ap.bf[2] * lo[MZ][r];
+                rt[MX][r] += ap.bf[0] * lo[EN][r];
+                rt[MY][r] += ap.bf[1] * lo[EN][r];
+                rt[MZ][r] += ap.bf[2] * lo[EN][r];
+            }
+        }
+        if (ap.rhs_only) {
+#pragma unroll
+            for (int f = 0; f < NQ; f++)
+#pragma unroll
+                for (int r = 0; r < NB; r++) lam_in[((long)r * NQ + f) * g.N + c] = rt[f][r];
+            return;
+        }
+        // lam_in = sum_k w_k acc_k + L'^T (lo + dt*rt)
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                const long o = ((long)r * NQ + f) * g.N + c;
+                double v = lo[f][r] + dt * rt[f][r];
+                if (ap.acc[0]) v += ap.w[0] * ap.acc[0][o];
+                if (ap.acc[1]) v += ap.w[1] * ap.acc[1][o];
+                rt[f][r] = v;
+            }
+        if (ap.q_raw) {
+            double raw[NQ][NB];
+            fz::load_modes<NB, NQ>(ap.q_raw, g, c, raw);
+            const LimFactors F = limit_factors<NB, QP, NQ>(raw, ap.lp);
+            limit_vjp<NB, NQ>(rt, F);
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) lam_in[((long)r * NQ + f) * g.N + c] = rt[f][r];
+    }
+    // per-block partial of d(lam.out)/dcoll
+    if (ap.partial) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dcoll += __shfl_xor(dcoll, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dcoll;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+            ap.partial[blockIdx.x] = s;
+        }
+    }
+}
+
+// adds the block partials to *acc (one block, fixed order)
+static __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, int n,
+                                                             double* __restrict__ acc) {
+    __shared__ double red[4];
+    double s = 0.;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) s += partial[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+        *acc += t;
+    }
+}
+
+}  // namespace adj
+}  // namespace hd

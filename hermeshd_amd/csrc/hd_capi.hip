@@ -218,6 +218,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
         (r = alloc(c, (void**)&c->scratch_dt, sizeof(double))) ||
         (r = alloc(c, (void**)&c->step_ctr, sizeof(unsigned long long))) ||
+        (r = alloc(c, (void**)&c->vjp_partial, blocks_for(c->N, 256) * sizeof(double))) ||
         (cfg->integrator == 3 &&
          (r = alloc(c, (void**)&c->stage_buf, (size_t)c->N * c->nQ * c->nb * sizeof(double))))) {
         std::string msg = g_last_error;
@@ -263,7 +264,7 @@ int hd_destroy(hd_ctx* c) {
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {c->dtab,     c->fl[0],    c->fl[1],     c->fl[2],      c->partial,    c->err,
                     c->bad,      c->scratch_dt, c->stage_buf, c->step_ctr, c->noise_f[0], c->noise_f[1],
-                    c->noise_f[2], c->noise_v};
+                    c->noise_f[2], c->noise_v,  c->vjp_partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -364,6 +365,81 @@ int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_
         if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
     }
     return advance_step(c, s);
+}
+
+int hd_stage(hd_ctx* c, int32_t rk_order, int32_t stage_index, double* q_in, const double* q_n, double* q_out,
+             const double* dt_dev, void* stream) {
+    if (!c || !q_in || !q_out || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (q_in == q_out) return fail(HD_ERR_INVALID, "q_in and q_out must differ");
+    int combine;
+    if (rk_order == 2 && (stage_index == 1 || stage_index == 2)) combine = stage_index == 1 ? 0 : 1;
+    else if (rk_order == 3 && stage_index >= 1 && stage_index <= 3) combine = stage_index == 1 ? 0 : stage_index;
+    else return fail(HD_ERR_INVALID, "rk_order must be 2 (stages 1-2) or 3 (stages 1-3)");
+    if (combine != 0 && !q_n) return fail(HD_ERR_INVALID, "q_n is required on combination stages");
+    if (q_n == q_out && combine != 0) return fail(HD_ERR_INVALID, "q_n and q_out must differ");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (stage_index == 1 && (rc = draw_noise(c, dt_dev, s))) return rc;
+    if ((rc = stage(c, q_in, combine ? q_n : q_in, q_out, dt_dev, combine, s, false, false))) return rc;
+    if (stage_index == rk_order) return advance_step(c, s);
+    return HD_OK;
+}
+
+int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* lam_in, const double* dt_dev,
+                 const hd_stage_vjp_args* args, void* stream) {
+    if (!c || !q_in || !lam_out || !lam_in || !dt_dev || !args) return fail(HD_ERR_INVALID, "NULL argument");
+    if (lam_in == lam_out || lam_in == q_in) return fail(HD_ERR_INVALID, "lam_in must differ from lam_out and q_in");
+    if (c->cfg.noise) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the stochastic stress");
+    Ops o;
+    if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
+    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels");
+    adj::AdjParams ap;
+    ap.dt = dt_dev;
+    ap.coll = c->cfg.coll;
+    ap.ivis = c->cfg.ivis;
+    ap.a = args->a;
+    ap.b = args->b;
+    for (int d = 0; d < 3; d++) ap.bf[d] = c->cfg.body_force[d];
+    ap.acc[0] = args->acc0; ap.acc[1] = args->acc1;
+    ap.w[0] = args->w0; ap.w[1] = args->w1;
+    ap.q_raw = c->cfg.limiter ? args->q_raw : nullptr;
+    ap.lp = c->lp;
+    ap.out = args->out;
+    ap.qn = args->qn;
+    if (ap.out && ap.a != 0. && !ap.qn) return fail(HD_ERR_INVALID, "qn is required with out when a != 0");
+    ap.partial = args->dcoll_dev ? c->vjp_partial : nullptr;
+    ap.rhs_only = 0;
+    hipStream_t s = (hipStream_t)stream;
+    o.stage_vjp(c, q_in, lam_out, lam_in, ap, s);
+    HIPCHK(hipGetLastError());
+    if (args->dcoll_dev) {
+        hipLaunchKernelGGL(adj::k_sum_partials, dim3(1), dim3(256), 0, s, c->vjp_partial, (int)blocks_for(c->N, 256),
+                           args->dcoll_dev);
+        HIPCHK(hipGetLastError());
+    }
+    return HD_OK;
+}
+
+int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void* stream) {
+    if (!c || !q || !lam || !out) return fail(HD_ERR_INVALID, "NULL argument");
+    if (out == lam || out == q) return fail(HD_ERR_INVALID, "out must differ from q and lam");
+    if (c->cfg.noise) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the stochastic stress");
+    Ops o;
+    if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
+    adj::AdjParams ap{};
+    ap.dt = c->scratch_dt;
+    ap.coll = c->cfg.coll;
+    ap.ivis = c->cfg.ivis;
+    ap.a = 0.;
+    ap.b = 1.;
+    for (int d = 0; d < 3; d++) ap.bf[d] = c->cfg.body_force[d];
+    ap.lp = c->lp;
+    ap.rhs_only = 1;
+    o.stage_vjp(c, q, lam, out, ap, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
 }
 
 int hd_set_step(hd_ctx* c, uint64_t step, void* stream) {

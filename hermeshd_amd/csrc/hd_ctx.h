@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/hermeshd_amd.h"
+#include "hd_adjoint.h"
 #include "hd_ensemble.h"
 
 struct hd_ctx {
@@ -32,6 +33,7 @@ struct hd_ctx {
     unsigned long long* bad = nullptr;
     double* scratch_dt = nullptr;  // holds 1.0: the time step hd_rhs gives the noise amplitude
     double* stage_buf = nullptr;   // third stage array of RK3 when the caller passes none
+    double* vjp_partial = nullptr; // per-block partial sums of the adjoint's parameter derivative
     // stochastic stress (cfg.noise): the step's field and the step counter
     unsigned long long* step_ctr = nullptr;
     double* noise_f[3] = {nullptr, nullptr, nullptr};
@@ -91,6 +93,8 @@ struct Ops {
     void (*fused)(hd_ctx*, const double*, const double*, double*, const StageParams&, hipStream_t);
     void (*ens_limit)(hd_ens*, double*, hipStream_t);
     void (*ens_stage)(hd_ens*, const double*, const double*, double*, const StageParams&, hipStream_t);
+    // nullptr when the configuration has no adjoint (Roe)
+    void (*stage_vjp)(hd_ctx*, const double*, const double*, double*, const adj::AdjParams&, hipStream_t);
 };
 
 // error text of the calling thread (hd_last_error) and the error return

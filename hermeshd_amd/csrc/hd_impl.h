@@ -51,7 +51,15 @@ struct Impl {
         hipLaunchKernelGGL((ens::k_ens_stage<NB, QP, NQ, RM, false>), dim3(e->grid), dim3(e->bs), 0, s, qin, qn,
                            qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise);
     }
-    static Ops ops() { return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage}; }
+    static void stage_vjp(hd_ctx* c, const double* q, const double* lam, double* lam_in, const adj::AdjParams& ap,
+                          hipStream_t s) {
+        if constexpr (RM != RIEMANN_ROE)
+            hipLaunchKernelGGL((adj::k_stage_vjp<NB, QP, NQ, RM>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q,
+                               lam, lam_in, c->grid, c->ph, ap);
+    }
+    static Ops ops() {
+        return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage, RM == RIEMANN_ROE ? nullptr : &stage_vjp};
+    }
 };
 
 template <int NB, int QP, int NQ>

@@ -102,8 +102,13 @@ HD_DEV void hllc_point(const Phys& ph, const double* QL, const double* QR, const
     const double SL = fmin(uL - cL, uR - cR);
     const double SR = fmax(uR + cR, uL + cL);
     // contact speed and star pressure, Eqs. (34),(36) of Batten 1997
-    double sm_num = rR * uR * (SR - uR) - rL * uL * (SL - uL);
-    sm_num = sm_num + pL - pR;
+    double sm_num;
+    {
+        // no contraction: the x-high outflow ghost makes this exactly zero
+#pragma clang fp contract(off)
+        sm_num = rR * uR * (SR - uR) - rL * uL * (SL - uL);
+        sm_num = sm_num + pL - pR;
+    }
     double sm_den = rR * (SR - uR) - rL * (SL - uL);
     if (sm_den == 0.0) sm_den = ph.rh_floor;
     const double SM = sm_num / sm_den;

@@ -78,6 +78,16 @@ __device__ __forceinline__ void flux(const Phys& ph, const double* Q, const Prim
     }
 }
 
+// Numerator of the HLLC contact speed, aR*uR - aL*uL + pL - pR, rounded
+// product by product: at a mirror state (the x-high outflow ghost reverses
+// the normal momentum) it is exactly zero, and a fused multiply-add would
+// leave a signed rounding residue that picks a different star state than
+// the reference's arithmetic does.
+__device__ __forceinline__ double contact_num(double aR, double uR, double aL, double uL, double pL, double pR) {
+#pragma clang fp contract(off)
+    return aR * uR - aL * uL + pL - pR;
+}
+
 // HLLC (Batten 1997 with Davis speeds) at one face point from primitives;
 // out[] holds the central flux on entry and keeps it when no branch applies.
 template <int DIR>
@@ -95,8 +105,7 @@ __device__ __forceinline__ void hllc(const Phys& ph, const double* QL, const Pri
     const double aR = rR * (SR - uR), aL = rL * (SL - uL);
     double sm_den = aR - aL;
     if (sm_den == 0.0) sm_den = ph.rh_floor;
-    const double sm_num = aR * uR - aL * uL + pL - pR;
-    const double SM = divide(sm_num, sm_den);
+    const double SM = divide(contact_num(aR, uR, aL, uL, pL, pR), sm_den);
     int sel = -1;
     if (SL > 0.0) sel = 0;
     if (SL <= 0.0 && 0.0 < SM) sel = 1;

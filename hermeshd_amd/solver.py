@@ -104,6 +104,45 @@ class Solver:
         _lib.check(_lib.lib().hd_step(self._h, self._ptr(q_io), self._ptr(q1, "q1"), p2,
                                       C.c_void_p(t.data_ptr()), C.c_void_p(dt.data_ptr()), _stream()))
 
+    def stage(self, rk_order: int, stage_index: int, q_in: torch.Tensor, q_n: torch.Tensor | None,
+              q_out: torch.Tensor, dt) -> None:
+        """One SSP-RK stage: limiter on q_in in place, q_out = a*q_n + b*E(q_in)."""
+        d = self._scalar(dt)
+        pn = self._ptr(q_n, "q_n") if q_n is not None else C.c_void_p(0)
+        _lib.check(_lib.lib().hd_stage(self._h, int(rk_order), int(stage_index), self._ptr(q_in, "q_in"), pn,
+                                       self._ptr(q_out, "q_out"), C.c_void_p(d.data_ptr()), _stream()))
+
+    def rhs_vjp(self, q: torch.Tensor, lam: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(dR/dq)^T lam at state q."""
+        out = torch.empty_like(q) if out is None else out
+        _lib.check(_lib.lib().hd_rhs_vjp(self._h, self._ptr(q), self._ptr(lam, "lam"), self._ptr(out, "out"),
+                                         _stream()))
+        return out
+
+    def stage_vjp(self, q_in, lam_out, lam_in, dt, a, b, acc=(), q_raw=None, out=None, qn=None, dcoll=None):
+        """Transpose of one stage at its limited input q_in applied to lam_out; writes lam_in.
+        acc: up to two (weight, array) pairs added to lam_in; dcoll: 0-d device tensor (+=)."""
+        d = self._scalar(dt)
+        args = _lib.StageVjpArgs()
+        args.a, args.b = float(a), float(b)
+        acc = list(acc)
+        if len(acc) > 2:
+            raise ValueError("at most two accumulated arrays")
+        for k, (w, arr) in enumerate(acc):
+            setattr(args, f"acc{k}", self._ptr(arr, f"acc{k}").value)
+            setattr(args, f"w{k}", float(w))
+        if q_raw is not None:
+            args.q_raw = self._ptr(q_raw, "q_raw").value
+        if out is not None:
+            args.out = self._ptr(out, "out").value
+        if qn is not None:
+            args.qn = self._ptr(qn, "qn").value
+        if dcoll is not None:
+            args.dcoll_dev = dcoll.data_ptr()
+        _lib.check(_lib.lib().hd_stage_vjp(self._h, self._ptr(q_in, "q_in"), self._ptr(lam_out, "lam_out"),
+                                           self._ptr(lam_in, "lam_in"), C.c_void_p(d.data_ptr()), C.byref(args),
+                                           _stream()))
+
     def set_step(self, step: int) -> None:
         """Step index that keys the stochastic stress (advanced by every update)."""
         _lib.check(_lib.lib().hd_set_step(self._h, int(step), _stream()))

@@ -104,6 +104,44 @@ int hd_update(hd_ctx *ctx, double *q_io, double *q1, double *q2, const double *d
 int hd_step(hd_ctx *ctx, double *q_io, double *q1, double *q2, double *t_dev, double *dt_dev,
             void *stream);
 
+/* One stage of an SSP-RK step: limiter on q_in (in place), then
+   q_out = a*q_n + b*E(q_in) with (a,b) of stage `stage_index` (1-based) of
+   SSP-RK2 (rk_order 2: (0,1), (1/2,1/2)) or SSP-RK3 (rk_order 3: (0,1),
+   (3/4,1/4), (1/3,2/3)).  q_out is not limited.  Replaces euler_step plus
+   the combination lines of RK2/RK3 (src/integrator.f90:62-122).  With noise
+   on, stage 1 draws the step's stress field and the last stage advances the
+   step index. */
+int hd_stage(hd_ctx *ctx, int32_t rk_order, int32_t stage_index, double *q_in, const double *q_n,
+             double *q_out, const double *dt_dev, void *stream);
+
+/* ---- adjoint (reverse mode) ------------------------------------------------
+   No counterpart in the reference, which has no adjoint: these give the
+   gradient of a run with respect to its initial state and the relaxation
+   rate coll (used for the Poiseuille-channel sensitivity).  Central and HLLC
+   face solvers; no noise. */
+
+/* out = (dR/dq)^T lam, R the right-hand side of hd_rhs at state q. */
+int hd_rhs_vjp(hd_ctx *ctx, const double *q, const double *lam, double *out, void *stream);
+
+typedef struct hd_stage_vjp_args {
+    double a, b;               /* the stage's weights: out = a*q_n + b*E(q_in) */
+    const double *acc0, *acc1; /* NULL or arrays added to lam_in with weights w0, w1 */
+    double w0, w1;
+    const double *q_raw;       /* stage input before the limiter (used when the limiter is on) */
+    const double *out, *qn;    /* forward stage output and q_n, for d/dcoll (out NULL: skip) */
+    double *dcoll_dev;         /* device double: += d(lam_out . out)/dcoll (NULL: skip) */
+} hd_stage_vjp_args;
+
+/* Transpose of one stage (hd_stage) at its (limited) input q_in, applied to
+   lam_out, the cotangent of its output:
+   lam_in = w0*acc0 + w1*acc1 + L'^T (B^T b lam_out + dt (dR/dq)^T B^T b lam_out),
+   B the implicit stress scaling (ivis 1), L' the limiter with the factors of
+   the forward pass at q_raw (active set frozen).  The cotangent a*lam_out of
+   q_n is left to the caller (pass it through acc0/acc1 of the stage that
+   produced q_n). */
+int hd_stage_vjp(hd_ctx *ctx, const double *q_in, const double *lam_out, double *lam_in,
+                 const double *dt_dev, const hd_stage_vjp_args *args, void *stream);
+
 /* Sets the step index that keys the stochastic stress (0 after hd_create),
    so that a restarted run draws the same noise.  Replaces the stream set-up
    of random_init (src/random.f90:36-39). */

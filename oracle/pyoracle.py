@@ -107,8 +107,8 @@ class Oracle:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
-            lib().orc_destroy(h)
+        if h and _lib is not None:
+            _lib.orc_destroy(h)
             self.h = None
 
     def _check(self, rc):
