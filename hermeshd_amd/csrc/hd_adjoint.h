@@ -265,6 +265,7 @@ __device__ __forceinline__ LimFactors limit_factors(const double (&m0)[MQ][NB], 
         F.zero = true;
         return F;
     }
+    if (fz::limiter_inactive<NB, MQ>(m0, lp)) return F;
     const auto& T = fz::face_tab<NB>();
     double rho[NB];
 #pragma unroll
@@ -337,6 +338,12 @@ __device__ __forceinline__ void limit_vjp(double (&lam)[MQ][NB], const LimFactor
 }
 
 // ------------------------------------------------------------- stage -----
+// grids up to this many cells use the four-wave split of the transpose (when
+// its three 64-cell partial sums fit the 160 KB of LDS)
+constexpr long kSplitCells = 1L << 17;
+template <int NB, int NQ>
+constexpr bool split_fits() { return 3L * NQ * NB * 64 * 8 <= 150L * 1024; }
+
 struct AdjParams {
     const double* dt;          // device time step
     double coll;
@@ -526,40 +533,18 @@ __device__ __forceinline__ void volume_vjp(const double (&m)[NQ][NB], const doub
             }
 }
 
+// Parts of the transpose of one cell: bit 0 volume + sources, bits 1..3 the
+// x, y, z face pairs.  Adds to rt; returns the cell's share of d/dcoll.
 template <int NB, int QP, int NQ, int RM>
-__global__ __launch_bounds__(256) void k_stage_vjp(const double* __restrict__ q, const double* __restrict__ lam,
-                                                   double* __restrict__ lam_in, Grid g, Phys ph, AdjParams ap) {
-    __shared__ double red[4];
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ double cell_vjp_parts(int parts, const double* __restrict__ q,
+                                                 const double* __restrict__ lam, const Grid& g, const Phys& ph,
+                                                 const AdjParams& ap, const fz::CellPos& cp, double dt, double faci,
+                                                 const double (&m)[NQ][NB], const double (&lo)[NQ][NB],
+                                                 double (&rt)[NQ][NB]) {
     double dcoll = 0.;
-    if (c < g.N) {
-        fz::CellPos cp;
-        cp.c = c;
-        cp.i = (int)(c % g.nx);
-        cp.j = (int)((c / g.nx) % g.ny);
-        cp.k = (int)(c / ((long)g.nx * g.ny));
-        const double dt = *ap.dt;
-        const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
-        double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
-        fz::load_modes<NB, NQ>(q, g, c, m);
-        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
-#pragma unroll
-        for (int f = 0; f < NQ; f++)
-#pragma unroll
-            for (int r = 0; r < NB; r++) rt[f][r] = 0.;
+    const long c = cp.c;
+    if (parts & 1) {
         volume_vjp<NB, QP, NQ>(m, lo, rt, ph, g.dxi, g.dyi, g.dzi);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
-        fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
         fz::fence();
         // sources: ivis 0 explicit relaxation -coll*P; body force rho*g, m.g
         if constexpr (NQ >= 11) {
@@ -593,47 +578,152 @@ __global__ __launch_bounds__(256) void k_stage_vjp(const double* __restrict__ q,
                 rt[MZ][r] += ap.bf[2] * lo[EN][r];
             }
         }
-        if (ap.rhs_only) {
-#pragma unroll
-            for (int f = 0; f < NQ; f++)
-#pragma unroll
-                for (int r = 0; r < NB; r++) lam_in[((long)r * NQ + f) * g.N + c] = rt[f][r];
-            return;
-        }
-        // lam_in = sum_k w_k acc_k + L'^T (lo + dt*rt)
-#pragma unroll
-        for (int f = 0; f < NQ; f++)
-#pragma unroll
-            for (int r = 0; r < NB; r++) {
-                const long o = ((long)r * NQ + f) * g.N + c;
-                double v = lo[f][r] + dt * rt[f][r];
-                if (ap.acc[0]) v += ap.w[0] * ap.acc[0][o];
-                if (ap.acc[1]) v += ap.w[1] * ap.acc[1][o];
-                rt[f][r] = v;
-            }
-        if (ap.q_raw) {
-            double raw[NQ][NB];
-            fz::load_modes<NB, NQ>(ap.q_raw, g, c, raw);
-            const LimFactors F = limit_factors<NB, QP, NQ>(raw, ap.lp);
-            limit_vjp<NB, NQ>(rt, F);
-        }
+    }
+    if (parts & 2) {
+        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        fz::fence();
+    }
+    if (parts & 4) {
+        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        fz::fence();
+    }
+    if (parts & 8) {
+        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        fz::fence();
+        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        fz::fence();
+    }
+    return dcoll;
+}
+
+// lam_in = sum_k w_k acc_k + L'^T (lo + dt*rt), or rt alone in rhs mode
+template <int NB, int QP, int NQ>
+__device__ __forceinline__ void cell_vjp_finish(double* __restrict__ lam_in, const Grid& g, const AdjParams& ap,
+                                                long c, double dt, const double (&lo)[NQ][NB], double (&rt)[NQ][NB]) {
+    if (ap.rhs_only) {
 #pragma unroll
         for (int f = 0; f < NQ; f++)
 #pragma unroll
             for (int r = 0; r < NB; r++) lam_in[((long)r * NQ + f) * g.N + c] = rt[f][r];
+        return;
     }
-    // per-block partial of d(lam.out)/dcoll
-    if (ap.partial) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) dcoll += __shfl_xor(dcoll, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dcoll;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
-            ap.partial[blockIdx.x] = s;
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            const long o = ((long)r * NQ + f) * g.N + c;
+            double v = lo[f][r] + dt * rt[f][r];
+            if (ap.acc[0]) v += ap.w[0] * ap.acc[0][o];
+            if (ap.acc[1]) v += ap.w[1] * ap.acc[1][o];
+            rt[f][r] = v;
         }
+    if (ap.q_raw) {
+        double raw[NQ][NB];
+        fz::load_modes<NB, NQ>(ap.q_raw, g, c, raw);
+        const LimFactors F = limit_factors<NB, QP, NQ>(raw, ap.lp);
+        limit_vjp<NB, NQ>(rt, F);
     }
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) lam_in[((long)r * NQ + f) * g.N + c] = rt[f][r];
+}
+
+// per-block partial of d(lam.out)/dcoll (every thread of the block calls it)
+__device__ __forceinline__ void block_partial(double v, double* partial) {
+    __shared__ double red[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+        partial[blockIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ fz::CellPos cell_pos(const Grid& g, long c) {
+    fz::CellPos cp;
+    cp.c = c;
+    cp.i = (int)(c % g.nx);
+    cp.j = (int)((c / g.nx) % g.ny);
+    cp.k = (int)(c / ((long)g.nx * g.ny));
+    return cp;
+}
+
+// One thread per cell.
+template <int NB, int QP, int NQ, int RM>
+__global__ __launch_bounds__(256) void k_stage_vjp(const double* __restrict__ q, const double* __restrict__ lam,
+                                                   double* __restrict__ lam_in, Grid g, Phys ph, AdjParams ap) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dcoll = 0.;
+    if (c < g.N) {
+        const fz::CellPos cp = cell_pos(g, c);
+        const double dt = *ap.dt;
+        const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+        double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
+        fz::load_modes<NB, NQ>(q, g, c, m);
+        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) rt[f][r] = 0.;
+        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(15, q, lam, g, ph, ap, cp, dt, faci, m, lo, rt);
+        cell_vjp_finish<NB, QP, NQ>(lam_in, g, ap, c, dt, lo, rt);
+    }
+    if (ap.partial) block_partial(dcoll, ap.partial);
+}
+
+// For small grids (latency-bound): a block of 4 waves takes 64 cells and each
+// wave one part (volume + sources, x, y, z faces); the face waves pass their
+// sums through LDS to the volume wave, which finishes the cells.  Four times
+// the parallelism of one thread per cell, at the same total work.
+template <int NB, int QP, int NQ, int RM>
+__global__ __launch_bounds__(256) void k_stage_vjp_split(const double* __restrict__ q,
+                                                         const double* __restrict__ lam, double* __restrict__ lam_in,
+                                                         Grid g, Phys ph, AdjParams ap) {
+    static_assert(split_fits<NB, NQ>(), "partial sums exceed LDS");
+    __shared__ double part_sum[3][NQ * NB][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long c = (long)blockIdx.x * 64 + lane;
+    double dcoll = 0.;
+    const double dt = *ap.dt;
+    const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+    double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) rt[f][r] = 0.;
+    const bool valid = c < g.N;
+    fz::CellPos cp;
+    if (valid) {
+        cp = cell_pos(g, c);
+        fz::load_modes<NB, NQ>(q, g, c, m);
+        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
+        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(1 << wave, q, lam, g, ph, ap, cp, dt, faci, m, lo, rt);
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++) part_sum[wave - 1][f * NB + r][lane] = rt[f][r];
+    }
+    __syncthreads();
+    if (wave == 0 && valid) {
+#pragma unroll
+        for (int f = 0; f < NQ; f++)
+#pragma unroll
+            for (int r = 0; r < NB; r++)
+                rt[f][r] += part_sum[0][f * NB + r][lane] + part_sum[1][f * NB + r][lane] +
+                            part_sum[2][f * NB + r][lane];
+        cell_vjp_finish<NB, QP, NQ>(lam_in, g, ap, c, dt, lo, rt);
+    }
+    if (ap.partial) block_partial(dcoll, ap.partial);
 }
 
 // adds the block partials to *acc (one block, fixed order)

@@ -218,7 +218,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
         (r = alloc(c, (void**)&c->scratch_dt, sizeof(double))) ||
         (r = alloc(c, (void**)&c->step_ctr, sizeof(unsigned long long))) ||
-        (r = alloc(c, (void**)&c->vjp_partial, blocks_for(c->N, 256) * sizeof(double))) ||
+        (r = alloc(c, (void**)&c->vjp_partial, blocks_for(c->N, 64) * sizeof(double))) ||
         (cfg->integrator == 3 &&
          (r = alloc(c, (void**)&c->stage_buf, (size_t)c->N * c->nQ * c->nb * sizeof(double))))) {
         std::string msg = g_last_error;
@@ -414,7 +414,7 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     o.stage_vjp(c, q_in, lam_out, lam_in, ap, s);
     HIPCHK(hipGetLastError());
     if (args->dcoll_dev) {
-        hipLaunchKernelGGL(adj::k_sum_partials, dim3(1), dim3(256), 0, s, c->vjp_partial, (int)blocks_for(c->N, 256),
+        hipLaunchKernelGGL(adj::k_sum_partials, dim3(1), dim3(256), 0, s, c->vjp_partial, c->vjp_blocks,
                            args->dcoll_dev);
         HIPCHK(hipGetLastError());
     }

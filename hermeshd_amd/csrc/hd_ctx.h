@@ -34,6 +34,7 @@ struct hd_ctx {
     double* scratch_dt = nullptr;  // holds 1.0: the time step hd_rhs gives the noise amplitude
     double* stage_buf = nullptr;   // third stage array of RK3 when the caller passes none
     double* vjp_partial = nullptr; // per-block partial sums of the adjoint's parameter derivative
+    int vjp_blocks = 0;            // blocks of the last transpose launch
     // stochastic stress (cfg.noise): the step's field and the step counter
     unsigned long long* step_ctr = nullptr;
     double* noise_f[3] = {nullptr, nullptr, nullptr};

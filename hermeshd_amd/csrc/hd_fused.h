@@ -315,6 +315,39 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
 // pressure step takes the minimum face pressure first and divides once:
 // (Pave-epsP)/(Pave-P) is monotone in P for P < epsP <= Pave, so this equals
 // the reference's minimum over points.
+// Exact early-out.  Every basis function is bounded by 1 in magnitude at the
+// face points (products of L0 = 1, L1 and L2 on [-1,1]), so
+// rho(p) >= rho_1 - sum_{r>1} |rho_r|, E(p) >= E_1 - sum |E_r|,
+// |m(p)|^2 <= sum_a (sum_r |m_a,r|)^2, and the face pressure is bounded below
+// by gm1*(E_lb - |m|^2_ub/(2 rho_lb)).  When both bounds clear the floors (with
+// a relative margin far above rounding), the limiter's minima clear them too
+// and it leaves the cell unchanged, so its face loops can be skipped.
+template <int NB, int MQ>
+__device__ __forceinline__ bool limiter_inactive(const double (&m)[MQ][NB], const LimParams& lp) {
+    double sr = 0., se = 0., mm[3] = {0., 0., 0.};
+#pragma unroll
+    for (int r = 1; r < NB; r++) {
+        sr += fabs(m[RH][r]);
+        se += fabs(m[EN][r]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) mm[a] += fabs(m[MX + a][r]);
+    }
+    const double rlb = m[RH][0] - sr;
+    if (!(rlb > lp.rh_floor + 1e-10 * (fabs(m[RH][0]) + sr))) return false;
+    double m2 = 0.;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double b = fabs(m[MX + a][0]) + mm[a];
+        m2 += b * b;
+    }
+    const double Pave =
+        lp.gm1 * (m[EN][0] - 0.5 * (m[MX][0] * m[MX][0] + m[MY][0] * m[MY][0] + m[MZ][0] * m[MZ][0]) / m[RH][0]);
+    if (Pave < lp.epsP) return false;
+    const double kin = 0.5 * m2 / rlb;
+    const double plb = lp.gm1 * (m[EN][0] - se - kin);
+    return plb > lp.epsP + 1e-10 * lp.gm1 * (fabs(m[EN][0]) + se + kin);
+}
+
 // Limits the modes m[0..4][*] of one cell in place; returns true if changed.
 template <int NB, int QP, int MQ>
 __device__ __forceinline__ bool limit_cell(double (&m)[MQ][NB], const LimParams& lp) {
@@ -326,6 +359,8 @@ __device__ __forceinline__ bool limit_cell(double (&m)[MQ][NB], const LimParams&
             for (int f = 0; f < 5; f++) m[f][r] = 0.0;
         m[RH][0] = lp.rh_floor;
         changed = true;
+    } else if (limiter_inactive<NB, MQ>(m, lp)) {
+        // no face point can reach a floor: the loops below would change nothing
     } else {
         const auto& T = face_tab<NB>();
         double rmin = 0.;

@@ -53,9 +53,20 @@ struct Impl {
     }
     static void stage_vjp(hd_ctx* c, const double* q, const double* lam, double* lam_in, const adj::AdjParams& ap,
                           hipStream_t s) {
-        if constexpr (RM != RIEMANN_ROE)
-            hipLaunchKernelGGL((adj::k_stage_vjp<NB, QP, NQ, RM>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q,
-                               lam, lam_in, c->grid, c->ph, ap);
+        if constexpr (RM != RIEMANN_ROE) {
+            // small grids are latency-bound: spread each cell over four waves
+            if constexpr (adj::split_fits<NB, NQ>()) {
+                if (c->N <= adj::kSplitCells) {
+                    c->vjp_blocks = (int)blocks_for(c->N, 64);
+                    hipLaunchKernelGGL((adj::k_stage_vjp_split<NB, QP, NQ, RM>), dim3(c->vjp_blocks), dim3(256), 0,
+                                       s, q, lam, lam_in, c->grid, c->ph, ap);
+                    return;
+                }
+            }
+            c->vjp_blocks = (int)blocks_for(c->N, 256);
+            hipLaunchKernelGGL((adj::k_stage_vjp<NB, QP, NQ, RM>), dim3(c->vjp_blocks), dim3(256), 0, s, q, lam,
+                               lam_in, c->grid, c->ph, ap);
+        }
     }
     static Ops ops() {
         return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage, RM == RIEMANN_ROE ? nullptr : &stage_vjp};
