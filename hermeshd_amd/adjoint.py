@@ -46,6 +46,14 @@ class AdjointRunner:
         self._solvers = {}
         self.last_stats = {}
 
+    def _dt(self, s: Solver) -> torch.Tensor:
+        """The time step as a device scalar (passed as a tensor, no fill per call)."""
+        t = getattr(self, "_dt_dev", None)
+        if t is None or t.device != s.device:
+            t = torch.full((), float(self.dt), dtype=torch.float64, device=s.device)
+            self._dt_dev = t
+        return t
+
     def solver(self, coll: float) -> Solver:
         key = float(coll)
         s = self._solvers.get(key)
@@ -63,8 +71,9 @@ class AdjointRunner:
         q1 = torch.zeros_like(q)
         q2 = torch.zeros_like(q) if self.order == 3 else None
         ck = [q.clone()]
+        dt = self._dt(s)
         for n in range(self.nsteps):
-            s.update(q, q1, q2, self.dt)
+            s.update(q, q1, q2, dt)
             if (n + 1) % self.checkpoint_every == 0 and n + 1 < self.nsteps:
                 ck.append(q.clone())
         return q, ck
@@ -75,13 +84,14 @@ class AdjointRunner:
         limited input, output)."""
         recs = []
         lim_on = self.cfg.limiter
+        dt = self._dt(s)
         for _ in range(nsteps):
             raws, lims, outs = [], [], []
             raw = q
             for k in range(self.order):
                 lim = raw.clone() if lim_on else raw   # the stage limits lim in place
                 out = torch.empty_like(raw)
-                s.stage(self.order, k + 1, lim, lims[0] if k > 0 else None, out, self.dt)
+                s.stage(self.order, k + 1, lim, lims[0] if k > 0 else None, out, dt)
                 raws.append(raw)
                 lims.append(lim)
                 outs.append(out)
@@ -93,6 +103,7 @@ class AdjointRunner:
     def _step_vjp(self, s: Solver, rec, lam: torch.Tensor, dcoll: torch.Tensor) -> torch.Tensor:
         raws, lims, outs = rec
         W = _WEIGHTS[self.order]
+        dt = self._dt(s)
         lam_out = lam
         qn_cot = []   # (weight, cotangent) pairs destined for the step input's limited state
         for k in reversed(range(self.order)):
@@ -101,7 +112,7 @@ class AdjointRunner:
             acc = []
             if k == 0:
                 acc = qn_cot
-            s.stage_vjp(lims[k], lam_out, lam_in, self.dt, a, b, acc=acc, q_raw=raws[k], out=outs[k],
+            s.stage_vjp(lims[k], lam_out, lam_in, dt, a, b, acc=acc, q_raw=raws[k], out=outs[k],
                         qn=lims[0] if a != 0.0 else None, dcoll=dcoll)
             if a != 0.0:
                 qn_cot.append((a, lam_out))

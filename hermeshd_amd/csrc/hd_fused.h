@@ -528,10 +528,18 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const
     }
 }
 
+// Workgroup b runs on XCD b mod 8.  Giving each XCD a contiguous eighth of the
+// cells keeps a cell's y and z neighbours (nx and nx*ny cells away) in the
+// same XCD's L2 instead of another's.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, G = gridDim.x;
+    return (G & 7u) == 0 ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+}
+
 template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
                                                double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     CellPos cp;
     cp.c = c;

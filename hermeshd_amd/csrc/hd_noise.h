@@ -51,9 +51,12 @@ __device__ __forceinline__ void normals4(uint64_t seed, uint32_t realization, ui
         const double a = ((double)u[2 * h] + 0.5) * two_m32;
         const double b = ((double)u[2 * h + 1] + 0.5) * two_m32;
         const double rad = sqrt(-2.0 * log(a));
-        const double ang = 6.283185307179586 * b;
-        out[2 * h] = rad * cos(ang);
-        out[2 * h + 1] = rad * sin(ang);
+        // cos/sin(2 pi b) through the pi-scaled forms (no large-argument
+        // reduction); equal to cos/sin(2 pi b) to an ulp or two
+        double sv, cv;
+        sincospi(2.0 * b, &sv, &cv);
+        out[2 * h] = rad * cv;
+        out[2 * h + 1] = rad * sv;
     }
 }
 
