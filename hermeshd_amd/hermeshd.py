@@ -30,18 +30,27 @@ import numpy as np
 import torch
 
 from ._lib import NonFiniteError  # noqa: F401  (re-exported)
+from .checkpoint import read_checkpoint, write_checkpoint
 from .config import HDConfig
 from .ics import set_ic
 from .solver import Solver
 
-_input = {"cfg": None, "icid": None, "tf": None, "ntout": 200, "clt": 2.0, "check_every": 1}
+_input = {"cfg": None, "icid": None, "tf": None, "ntout": 200, "clt": 2.0, "check_every": 1,
+          "iread": 0, "iwrite": 0, "fpre": "Qout", "data_dir": "data", "resuming": True, "mpi": (1, 1, 1)}
 _state = {"solver": None, "q1_dev": None, "steps": 0}
 
 
 def configure(cfg: HDConfig, icid: int | None = None, tf: float | None = None, ntout: int = 200,
-              clt: float = 2.0, check_every: int = 1) -> None:
-    """Set the run's switches (the reference's input.f90 module)."""
-    _input.update(cfg=cfg, icid=icid, tf=tf, ntout=ntout, clt=clt, check_every=max(0, int(check_every)))
+              clt: float = 2.0, check_every: int = 1, iread: int = 0, iwrite: int = 0, fpre: str = "Qout",
+              data_dir: str = "data", resuming: bool = True, mpi=(1, 1, 1)) -> None:
+    """Set the run's switches (the reference's input.f90 module).  iread > 0
+    starts from checkpoint dump iread (keeping its t, dt, nout when resuming);
+    iwrite = 1 writes a checkpoint at every output time, alternating dumps 1
+    and 2, split over `mpi` = (mpi_nx, mpi_ny, mpi_nz) rank files as the
+    reference would (src/hermeshd.f90:206-266, src/initialize.f90:394-565)."""
+    _input.update(cfg=cfg, icid=icid, tf=tf, ntout=ntout, clt=clt, check_every=max(0, int(check_every)),
+                  iread=int(iread), iwrite=int(iwrite), fpre=fpre, data_dir=data_dir, resuming=bool(resuming),
+                  mpi=tuple(mpi))
 
 
 def _cfg() -> HDConfig:
@@ -101,8 +110,19 @@ class Hermeshd:
         _set_scalar(t1, time.time())
         if _input["tf"] is not None:
             _set_scalar(dtout, _input["tf"] / _input["ntout"])
-        if _input["icid"] is not None:
+        q0 = None
+        if _input["iread"]:
+            # set_ic_from_file (src/initialize.f90:394-439)
+            Q, tp, dtp, noutp, _ = read_checkpoint(_input["data_dir"], _input["fpre"], _input["iread"],
+                                                   (cfg.nx, cfg.ny, cfg.nz), cfg.nQ, cfg.nbasis)
+            q0 = torch.from_numpy(np.ascontiguousarray(Q.T))
+            if _input["resuming"]:
+                _set_scalar(t, tp)
+                _set_scalar(dt, dtp)
+                _set_scalar(nout, noutp)
+        elif _input["icid"] is not None:
             q0 = torch.from_numpy(set_ic(cfg, _input["icid"]))
+        if q0 is not None:
             if isinstance(q_io, torch.Tensor):
                 q_io.copy_(q0.to(q_io.device))
             else:
@@ -141,12 +161,18 @@ class Hermeshd:
     @staticmethod
     def generate_output(q_r, t, dt, t1, dtout, nout):
         """Output scheduling of generate_output (src/hermeshd.f90:206-266):
-        advances nout when t passes dtout*nout.  Writing VTK/checkpoint files
-        is not part of this package."""
+        advances nout when t passes dtout*nout and, with iwrite = 1, writes a
+        checkpoint in the reference's format.  VTK output is not part of this
+        package."""
         tv, dto, n = _get_scalar(t), _get_scalar(dtout), int(np.asarray(nout)) if not isinstance(nout, torch.Tensor) else int(nout.item())
         if dto > 0 and tv > dto * n:
             _set_scalar(nout, n + 1)
             _set_scalar(t1, time.time())
+            if _input["iwrite"] == 1:
+                # writeQ with an odd/even dump id, so the last two sets are kept
+                ioe = 2 - (n + 1) % 2
+                write_checkpoint(_input["data_dir"], _input["fpre"], ioe, q_r, tv, _get_scalar(dt), n + 1,
+                                 mpi=_input["mpi"])
 
     @staticmethod
     def cleanup(t_start):

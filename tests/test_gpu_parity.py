@@ -293,3 +293,39 @@ def test_stage_entry_point_composes_the_update(torch_mod, name):
         s.stage(order, k, x, lims[0] if k > 1 else None, out, 1e-3)
         x = out
     assert torch_mod.equal(a, x)
+
+
+def test_front_end_checkpoint_restart(torch_mod, tmp_path):
+    """iwrite=1 writes reference-format checkpoints at output times; a run
+    restarted from one (iread, resuming) continues exactly as the
+    uninterrupted run."""
+    from hermeshd_amd import hermeshd as H
+    cfg = sod_1d(32)
+    H.configure(cfg, icid=3, tf=0.02, ntout=4, iwrite=1, data_dir=str(tmp_path), mpi=(2, 1, 1))
+    Q = np.zeros((cfg.nx, cfg.ny, cfg.nz, cfg.nQ, cfg.nbasis), order="F")
+    Q1, Q2 = np.zeros_like(Q), np.zeros_like(Q)
+    t, dt, t1, t_start, dtout = (np.array(0.0) for _ in range(5))
+    nout = np.array(0)
+    H.hermeshd.setup(Q, t, dt, t1, t_start, dtout, nout, None)
+    saved = None
+    while t < 0.02:
+        H.hermeshd.step(Q, Q1, Q2, t, dt)
+        n_before = int(nout)
+        H.hermeshd.generate_output(Q, t, dt, t1, dtout, nout)
+        if int(nout) == 2 and n_before == 1:
+            saved = (Q.copy(), float(t))
+    final = Q.copy()
+    H.hermeshd.cleanup(t_start)
+    assert saved is not None and (tmp_path / "Qout_p0001_d0002.dat").exists()
+    # restart from dump 2 (nout = 2) and run to the same end
+    H.configure(cfg, tf=0.02, ntout=4, iread=2, data_dir=str(tmp_path))
+    R = np.zeros_like(Q)
+    t, dt, t1, t_start, dtout = (np.array(0.0) for _ in range(5))
+    nout = np.array(0)
+    H.hermeshd.setup(R, t, dt, t1, t_start, dtout, nout, None)
+    assert np.array_equal(R, saved[0]) and float(t) == saved[1] and int(nout) == 2
+    while t < 0.02:
+        H.hermeshd.step(R, Q1, Q2, t, dt)
+        H.hermeshd.generate_output(R, t, dt, t1, dtout, nout)
+    H.hermeshd.cleanup(t_start)
+    assert np.array_equal(R, final)
