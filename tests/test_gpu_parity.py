@@ -312,18 +312,18 @@ def test_front_end_checkpoint_restart(torch_mod, tmp_path):
         H.hermeshd.step(Q, Q1, Q2, t, dt)
         n_before = int(nout)
         H.hermeshd.generate_output(Q, t, dt, t1, dtout, nout)
-        if int(nout) == 2 and n_before == 1:
-            saved = (Q.copy(), float(t))
+        if int(nout) != n_before and int(nout) % 2 == 0:   # this output wrote dump 2
+            saved = (Q.copy(), float(t), int(nout))
     final = Q.copy()
     H.hermeshd.cleanup(t_start)
     assert saved is not None and (tmp_path / "Qout_p0001_d0002.dat").exists()
-    # restart from dump 2 (nout = 2) and run to the same end
+    # restart from dump 2 (the last even output) and run to the same end
     H.configure(cfg, tf=0.02, ntout=4, iread=2, data_dir=str(tmp_path))
     R = np.zeros_like(Q)
     t, dt, t1, t_start, dtout = (np.array(0.0) for _ in range(5))
     nout = np.array(0)
     H.hermeshd.setup(R, t, dt, t1, t_start, dtout, nout, None)
-    assert np.array_equal(R, saved[0]) and float(t) == saved[1] and int(nout) == 2
+    assert np.array_equal(R, saved[0]) and float(t) == saved[1] and int(nout) == saved[2]
     while t < 0.02:
         H.hermeshd.step(R, Q1, Q2, t, dt)
         H.hermeshd.generate_output(R, t, dt, t1, dtout, nout)
