@@ -71,10 +71,13 @@ class HDConfig:
             raise ValueError(f"(nbasis, iquad) = {(self.nbasis, self.iquad)} not in {sorted(VALID_BASIS)}")
         if self.nQ not in (5, 11):
             raise ValueError("nQ must be 5 (hydro fields) or 11 (hydro + stress, reference layout)")
-        if self.ivis not in (0, 1):
-            raise ValueError("ivis must be 0 (explicit) or 1 (semi-implicit stress relaxation)")
-        if self.ivis == 1 and self.nQ != 11:
-            raise ValueError("ivis=1 needs the stress fields (nQ=11)")
+        if self.ivis not in (0, 1, 2):
+            raise ValueError("ivis must be 0 (explicit), 1 (semi-implicit stress relaxation) or 2 "
+                             "(full 10-moment equations)")
+        if self.ivis >= 1 and self.nQ != 11:
+            raise ValueError("ivis=1,2 need the stress fields (nQ=11)")
+        if self.ivis == 2 and self.noise:
+            raise ValueError("ivis=2 has no stochastic stress")
         if self.noise and self.nQ != 11:
             raise ValueError("stochastic stress needs nQ=11")
         if self.ieos != 1:

@@ -171,8 +171,9 @@ orc_ctx *orc_create(const orc_config *cfg) {
              (cfg->nbasis == 10 && cfg->iquad == 3);
     if (!ok) { set_err(-1, "unsupported (nbasis, iquad)"); return NULL; }
     if (cfg->nQ != 5 && cfg->nQ != 11) { set_err(-1, "nQ must be 5 or 11"); return NULL; }
-    if (cfg->ivis != 0 && cfg->ivis != 1) { set_err(-1, "ivis must be 0 or 1"); return NULL; }
-    if (cfg->ivis == 1 && cfg->nQ != 11) { set_err(-1, "ivis=1 needs nQ=11"); return NULL; }
+    if (cfg->ivis < 0 || cfg->ivis > 2) { set_err(-1, "ivis must be 0, 1 or 2"); return NULL; }
+    if (cfg->ivis >= 1 && cfg->nQ != 11) { set_err(-1, "ivis=1,2 need nQ=11"); return NULL; }
+    if (cfg->ivis == 2 && cfg->noise) { set_err(-1, "ivis=2 has no stochastic stress"); return NULL; }
     if (cfg->noise && cfg->nQ != 11) { set_err(-1, "noise needs nQ=11"); return NULL; }
     for (int d = 0; d < 3; d++) {
         int lo = cfg->bc[2 * d], hi = cfg->bc[2 * d + 1];
@@ -307,6 +308,51 @@ static void flux_points_s(const orc_ctx *c, const double *qp, double *fp, int di
         double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
         double P = c->aindm1 * (Ener - 0.5 * dni * smsq);
         if (P < c->P_floor) P = c->P_floor;
+        if (c->c.ivis == 2) {
+            /* full 10-moment model (src/flux.f90:64-66,89-117,134-155,172-191):
+               the stress fields hold the second moments E_ij = rho u_i u_j + P_ij;
+               the momentum flux swaps the isotropic 10-moment pressure P_10 for
+               the equation-of-state pressure P_5 */
+            double P_10 = (1. / 3.) * (P_xx + P_yy + P_zz - dni * smsq);
+            if (P_10 < c->P_floor) P_10 = c->P_floor;
+            const double P_5 = P;
+            switch (dir) {
+            case 1:
+                F[RH] = M_x;
+                F[MX] = P_xx - P_10 + P_5; F[MY] = P_xy; F[MZ] = P_xz;
+                F[EN] = (Ener + P) * vx + P_xx * vx + P_xy * vy + P_xz * vz;
+                F[PXX] = vx * (3 * P_xx - 2 * vx * M_x);
+                F[PYY] = 2 * vy * (P_xy - vy * M_x) + vx * P_yy;
+                F[PZZ] = 2 * vz * (P_xz - vz * M_x) + vx * P_zz;
+                F[PXY] = 2 * vx * (P_xy - vy * M_x) + vy * P_xx;
+                F[PXZ] = 2 * vx * (P_xz - vz * M_x) + vz * P_xx;
+                F[PYZ] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vy * vz * M_x;
+                break;
+            case 2:
+                F[RH] = M_y;
+                F[MX] = P_xy; F[MY] = P_yy - P_10 + P_5; F[MZ] = P_yz;
+                F[EN] = (Ener + P) * vy + P_yy * vy + P_xy * vx + P_yz * vz;
+                F[PXX] = 2 * vx * (P_xy - vx * M_y) + vy * P_xx;
+                F[PYY] = vy * (3 * P_yy - 2 * vy * M_y);
+                F[PZZ] = 2 * vz * (P_yz - vz * M_y) + vy * P_zz;
+                F[PXY] = 2 * vy * (P_xy - vx * M_y) + vx * P_yy;
+                F[PXZ] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vx * vz * M_y;
+                F[PYZ] = 2 * vy * (P_yz - vz * M_y) + vz * P_yy;
+                break;
+            default:
+                F[RH] = M_z;
+                F[MX] = P_xz; F[MY] = P_yz; F[MZ] = P_zz - P_10 + P_5;
+                F[EN] = (Ener + P) * vz + P_zz * vz + P_xz * vx + P_yz * vy;
+                F[PXX] = 2 * vx * (P_xz - vx * M_z) + vz * P_xx;
+                F[PYY] = 2 * vy * (P_yz - vy * M_z) + vz * P_yy;
+                F[PZZ] = vz * (3 * P_zz - 2 * vz * M_z);
+                F[PXY] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vx * vy * M_z;
+                F[PXZ] = 2 * vz * (P_xz - vx * M_z) + vx * P_zz;
+                F[PYZ] = 2 * vz * (P_yz - vy * M_z) + vy * P_zz;
+                break;
+            }
+            continue;
+        }
         switch (dir) {
         case 1:
             F[RH] = M_x;
@@ -350,11 +396,13 @@ int orc_flux_points(const orc_ctx *c, const double *qp, double *fp, int dir, int
     return 0;
 }
 
-/* cfcal (src/flux.f90:1029-1072), ivis in {0,1}, ieos = 1 */
+/* cfcal (src/flux.f90:1029-1072), ieos = 1; ivis 2 takes the pressure from
+   the trace of the second moments */
 double orc_cfcal(const orc_ctx *c, const double *Q, int dir) {
     double dn = Q[RH], dni = 1. / dn;
     double vx = Q[MX] * dni, vy = Q[MY] * dni, vz = Q[MZ] * dni;
     double P = c->aindm1 * (Q[EN] - 0.5 * dn * (vx * vx + vy * vy + vz * vz));
+    if (c->c.ivis == 2) P = (1. / 3.) * (Q[PXX] + Q[PYY] + Q[PZZ] - dn * (vx * vx + vy * vy + vz * vz));
     double cs = sqrt(c->aindex * P * dni);
     double v = (dir == 1) ? vx : (dir == 2) ? vy : vz;
     return fabs(v) + cs;
@@ -520,10 +568,17 @@ static void roe_face(const orc_ctx *c, const double *qlr, const double *flr, dou
 int orc_riemann_face(const orc_ctx *c, const double *qlr, const double *flr, double *fout, int dir,
                      int kind) {
     const int nf = c->nface, nQ = c->nQ;
+    /* cfr: local Lax-Friedrichs speeds, set only when ivis == 2
+       (src/flux.f90:243-254); otherwise the module array is zero */
+    double cfr[NFMAX];
+    for (int p = 0; p < nf; p++)
+        cfr[p] = c->c.ivis == 2 ? fmax(orc_cfcal(c, qlr + (long)p * nQ, dir), orc_cfcal(c, qlr + (long)(p + nf) * nQ, dir))
+                               : 0.0;
     for (int e = 0; e < nQ; e++)
         for (int p = 0; p < nf; p++) {
             int pp = p + nf;
-            fout[(long)p * nQ + e] = 0.5 * (flr[(long)p * nQ + e] + flr[(long)pp * nQ + e]);
+            fout[(long)p * nQ + e] = 0.5 * (flr[(long)p * nQ + e] + flr[(long)pp * nQ + e]) -
+                                     0.5 * cfr[p] * (qlr[(long)pp * nQ + e] - qlr[(long)p * nQ + e]);
         }
     if (kind == ORC_FLUX_HLLC) {
         double fh[NFMAX * 5];
@@ -942,6 +997,19 @@ int orc_source(orc_ctx *c, const double *q, double *source) {
                     for (int f = 0; f < nQ; f++) src[p * nQ + f] = 0;
                     if (nQ >= 11 && g->ivis == 0)
                         for (int f = PXX; f <= PYZ; f++) src[p * nQ + f] = -g->coll * Qin[f];
+                    if (nQ >= 11 && g->ivis == 2) {
+                        /* explicit 10-moment relaxation source (src/sources.f90:55-73) */
+                        double dn = Qin[RH], dni = 1. / dn;
+                        double vx = Qin[MX] * dni, vy = Qin[MY] * dni, vz = Qin[MZ] * dni;
+                        double vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+                        double colldn = g->coll * dn;
+                        src[p * nQ + PXX] = colldn * (2 * vx2 - vy2 - vz2) * (1. / 3.);
+                        src[p * nQ + PYY] = colldn * (2 * vy2 - vz2 - vx2) * (1. / 3.);
+                        src[p * nQ + PZZ] = colldn * (2 * vz2 - vx2 - vy2) * (1. / 3.);
+                        src[p * nQ + PXY] = colldn * vx * vy;
+                        src[p * nQ + PXZ] = colldn * vx * vz;
+                        src[p * nQ + PYZ] = colldn * vy * vz;
+                    }
                 }
                 for (int r = 0; r < nb; r++)
                     for (int f = 0; f < nQ; f++) {
@@ -988,6 +1056,22 @@ int orc_euler_step(orc_ctx *c, double *q_in, double *q_out, double dt) {
                 if (g->ivis == 1 && f >= PXX) v = faci * v;
                 q_out[o] = v;
             }
+    if (g->ivis == 2) {
+        /* backward-Euler relaxation of the second moments towards their
+           trace (src/integrator.f90:181-228) */
+        for (int r = 0; r < nb; r++)
+            for (long e = 0; e < n; e++) {
+#define O(f) (e + n * ((f) + (long)nQ * r))
+                double Q_sum = g->coll * dt * (q_in[O(PXX)] + q_in[O(PYY)] + q_in[O(PZZ)]) * (1. / 3.);
+                double glf_sum = g->coll * dt * dt * (c->glf[O(PXX)] + c->glf[O(PYY)] + c->glf[O(PZZ)]) * (1. / 3.);
+                double src_sum = g->coll * dt * dt * (c->src[O(PXX)] + c->src[O(PYY)] + c->src[O(PZZ)]) * (1. / 3.);
+                for (int f = PXX; f <= PZZ; f++)
+                    q_out[O(f)] = faci * (q_in[O(f)] - dt * c->glf[O(f)] + dt * c->src[O(f)] + Q_sum - glf_sum + src_sum);
+                for (int f = PXY; f <= PYZ; f++)
+                    q_out[O(f)] = (q_in[O(f)] - dt * c->glf[O(f)] + dt * c->src[O(f)]) * faci;
+#undef O
+            }
+    }
     for (int f = 0; f < nQ; f++)
         for (long e = 0; e < n; e++) {
             double v = q_out[e + n * f];

@@ -22,7 +22,7 @@ enum { ORC_FLUX_CENTRAL = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_ROE = 2 };
 typedef struct orc_config {
     int nx, ny, nz;
     int nbasis, iquad, nQ;       /* (nbasis,iquad) in {(4,2),(8,2),(10,3)}; nQ in {5,11} */
-    int ivis, ieos;              /* ivis in {0,1}; ieos = 1 (ideal gas) */
+    int ivis, ieos;              /* ivis in {0,1,2}; ieos = 1 (ideal gas) */
     int riemann;                 /* ORC_FLUX_* */
     int limiter;                 /* 1: positivity limiter on every stage input */
     int integrator;              /* 2 = heun (SSP-RK2), 3 = shu-osher (SSP-RK3) */

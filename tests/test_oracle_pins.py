@@ -482,3 +482,104 @@ def test_noise_keyed_by_step(oracle_mod):
     assert not np.allclose(a, o2.rhs(q))
     o3 = oracle_mod.Oracle(cfg.replace(seed=100).as_ints())
     assert not np.allclose(a, o3.rhs(q))
+
+
+# ------------------------------------------------- full 10-moment (ivis 2) ----
+def _ten_moment_cfg(**kw):
+    base = dict(nx=5, ny=4, nz=3, nbasis=4, iquad=2, nQ=11, ivis=2, coll=4.0, colvis=0.0, riemann="hllc",
+                lx=1.0, ly=0.8, lz=0.6, T_floor=1e-4, limiter=False)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+def _ten_moment_states(rng, n):
+    """Random states whose stress fields are second moments E_ij = rho u_i u_j + P_ij
+    with a positive-definite P_ij."""
+    U = np.zeros((n, 11))
+    rho = rng.uniform(0.5, 1.5, n)
+    u = rng.uniform(-0.4, 0.4, (n, 3))
+    A = rng.standard_normal((n, 3, 3)) * 0.3
+    P = np.einsum("nij,nkj->nik", A, A) + np.eye(3) * rng.uniform(0.5, 1.5, (n, 1, 1))
+    E = rho[:, None, None] * np.einsum("ni,nj->nij", u, u) + P
+    U[:, 0] = rho
+    U[:, 1:4] = rho[:, None] * u
+    U[:, 4] = np.trace(P, axis1=1, axis2=2) / 2 + 0.5 * rho * (u ** 2).sum(1)
+    U[:, 5], U[:, 6], U[:, 7] = E[:, 0, 0], E[:, 1, 1], E[:, 2, 2]
+    U[:, 8], U[:, 9], U[:, 10] = E[:, 0, 1], E[:, 0, 2], E[:, 1, 2]
+    return U, rho, u, E
+
+
+def test_ten_moment_flux_matches_tensor_form(oracle_mod):
+    """The explicit per-direction 10-moment stress fluxes equal the tensor form
+    F_d(E_ij) = u_i E_jd + u_j E_id + u_d E_ij - 2 rho u_i u_j u_d, and the momentum
+    flux is E_id with the trace pressure swapped for the gas pressure."""
+    cfg = _ten_moment_cfg()
+    o = oracle_mod.Oracle(cfg.as_ints())
+    rng = np.random.default_rng(21)
+    U, rho, u, E = _ten_moment_states(rng, 40)
+    idx = {(0, 0): 5, (1, 1): 6, (2, 2): 7, (0, 1): 8, (1, 0): 8, (0, 2): 9, (2, 0): 9, (1, 2): 10, (2, 1): 10}
+    for d in range(3):
+        F = o.flux_points(U, d + 1)
+        np.testing.assert_allclose(F[:, 0], U[:, 1 + d], rtol=1e-14)
+        for i in range(3):
+            for j in range(i, 3):
+                want = (u[:, i] * E[:, j, d] + u[:, j] * E[:, i, d] + u[:, d] * E[:, i, j]
+                        - 2 * rho * u[:, i] * u[:, j] * u[:, d])
+                np.testing.assert_allclose(F[:, idx[(i, j)]], want, rtol=1e-12, atol=1e-13)
+        trP = np.trace(E, axis1=1, axis2=2) - rho * (u ** 2).sum(1)
+        p_gas = (cfg.gamma - 1) * (U[:, 4] - 0.5 * rho * (u ** 2).sum(1))
+        for i in range(3):
+            want = E[:, i, d] + (p_gas - trP / 3 if i == d else 0.0)
+            np.testing.assert_allclose(F[:, 1 + i], want, rtol=1e-12, atol=1e-13)
+
+
+def test_ten_moment_relaxes_to_isotropy(oracle_mod):
+    """Uniform state at rest with an anisotropic pressure tensor: the implicit
+    update keeps the trace of E and relaxes its deviatoric part by exactly
+    1/(1 + dt*coll) per step (no fluxes, zero source at rest)."""
+    cfg = _ten_moment_cfg(bc=("periodic",) * 6)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = ics.uniform(cfg, rho=1.0, T=1.0)
+    Pd = np.array([1.6, 0.9, 0.5, 0.2, -0.1, 0.05])     # xx, yy, zz, xy, xz, yz
+    q[0, 5:] = Pd[:, None, None, None]
+    dt, coll = 0.01, cfg.coll
+    out = np.zeros_like(q)
+    o.euler_step(q, out, dt)
+    tr = Pd[:3].sum()
+    dev = Pd.copy()
+    dev[:3] -= tr / 3
+    want = dev / (1 + dt * coll)
+    want[:3] += tr / 3
+    np.testing.assert_allclose(out[0, 5:, 0, 0, 0], want, rtol=1e-13)
+    np.testing.assert_allclose(out[0, :5], q[0, :5], rtol=1e-13, atol=1e-14)
+
+
+def test_ten_moment_lax_friedrichs_dissipation(oracle_mod):
+    """With ivis 2 the face flux of every field carries the local Lax-Friedrichs
+    term -0.5*max(|u_d|+c)*(Q_R - Q_L), c from the trace pressure; with the
+    central solver nothing overrides it."""
+    cfg = _ten_moment_cfg(riemann="central")
+    o = oracle_mod.Oracle(cfg.as_ints())
+    rng = np.random.default_rng(5)
+    UL, rhoL, uL, EL = _ten_moment_states(rng, 4)
+    UR, rhoR, uR, ER = _ten_moment_states(rng, 4)
+    for d in range(3):
+        out = o.riemann_face(np.vstack([UL, UR]), d + 1, 0)
+        FL, FR = o.flux_points(UL, d + 1), o.flux_points(UR, d + 1)
+        pL = (np.trace(EL, axis1=1, axis2=2) - rhoL * (uL ** 2).sum(1)) / 3
+        pR = (np.trace(ER, axis1=1, axis2=2) - rhoR * (uR ** 2).sum(1)) / 3
+        c = np.maximum(np.abs(uL[:, d]) + np.sqrt(cfg.gamma * pL / rhoL), np.abs(uR[:, d]) + np.sqrt(cfg.gamma * pR / rhoR))
+        np.testing.assert_allclose(out, 0.5 * (FL + FR) - 0.5 * c[:, None] * (UR - UL), rtol=1e-12, atol=1e-13)
+
+
+def test_ten_moment_conserves_totals(oracle_mod):
+    cfg = _ten_moment_cfg(bc=("periodic",) * 6)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    rng = np.random.default_rng(8)
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.2, -0.1, 0.1))
+    U, _, _, _ = _ten_moment_states(rng, cfg.ncell)
+    q[0] = U.T.reshape(11, cfg.nz, cfg.ny, cfg.nx)
+    q[1:] = 0.01 * rng.standard_normal(q[1:].shape)
+    r = o.rhs(q)
+    tot = r[0].reshape(11, -1).sum(axis=1)
+    assert np.abs(tot[:5]).max() < 1e-12 * np.abs(r[0]).max()
