@@ -42,8 +42,9 @@ int validate(const hd_config* c) {
                     (c->nbasis == 10 && c->iquad == 3);
     if (!ok) return fail(HD_ERR_INVALID, "(nbasis, iquad) must be (4,2), (8,2) or (10,3)");
     if (c->nQ != 5 && c->nQ != 11) return fail(HD_ERR_INVALID, "nQ must be 5 or 11");
-    if (c->ivis != 0 && c->ivis != 1) return fail(HD_ERR_INVALID, "ivis must be 0 or 1");
-    if (c->ivis == 1 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=1 needs nQ=11");
+    if (c->ivis < 0 || c->ivis > 2) return fail(HD_ERR_INVALID, "ivis must be 0, 1 or 2");
+    if (c->ivis >= 1 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=1,2 need nQ=11");
+    if (c->ivis == 2 && c->noise) return fail(HD_ERR_INVALID, "ivis=2 has no stochastic stress");
     if (c->ieos != 1) return fail(HD_ERR_INVALID, "only ieos=1 (ideal gas) is supported");
     if (c->riemann < 0 || c->riemann > 2) return fail(HD_ERR_INVALID, "riemann must be 0, 1 or 2");
     if (c->integrator != 2 && c->integrator != 3) return fail(HD_ERR_INVALID, "integrator must be 2 or 3");
@@ -202,6 +203,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     ph.gamma = cfg->gamma; ph.gm1 = cfg->gamma - 1.;
     ph.P_floor = cfg->T_floor * cfg->rh_floor; ph.rh_floor = cfg->rh_floor;
     ph.colvis = cfg->colvis; ph.c2d3cv = (2. / 3.) * cfg->colvis; ph.c4d3cv = (4. / 3.) * cfg->colvis;
+    ph.ivis = cfg->ivis;
     c->lp.rh_floor = cfg->rh_floor; c->lp.epsP = cfg->rh_floor * cfg->T_floor; c->lp.gm1 = ph.gm1;
     build_tables(c->nb, c->q, &c->htab);
     int r = HD_OK;
@@ -250,7 +252,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
     const char* kern = std::getenv("HD_KERNELS");
-    c->generic = kern && std::strcmp(kern, "generic") == 0;
+    // the full 10-moment equations (ivis 2) run on the table-driven kernels only
+    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2;
     if (c->generic && cfg->noise) {
         hd_destroy(c);
         return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
@@ -393,7 +396,7 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
-    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels");
+    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
     adj::AdjParams ap;
     ap.dt = dt_dev;
     ap.coll = c->cfg.coll;
@@ -428,6 +431,7 @@ int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void*
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
+    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
     adj::AdjParams ap{};
     ap.dt = c->scratch_dt;
     ap.coll = c->cfg.coll;

@@ -181,6 +181,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     if (nreal < 1) return fail(HD_ERR_INVALID, "nreal must be >= 1");
     int rc = validate(base);
     if (rc) return rc;
+    if (base->ivis == 2) return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (ivis 0 or 1)");
     for (int r = 0; r < nreal; r++) {
         hd_config c = *base;
         c.nx = dims[3 * r]; c.ny = dims[3 * r + 1]; c.nz = dims[3 * r + 2];
@@ -205,6 +206,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     ph.gamma = base->gamma; ph.gm1 = base->gamma - 1.;
     ph.P_floor = base->T_floor * base->rh_floor; ph.rh_floor = base->rh_floor;
     ph.colvis = base->colvis; ph.c2d3cv = (2. / 3.) * base->colvis; ph.c4d3cv = (4. / 3.) * base->colvis;
+    ph.ivis = base->ivis;
     e->lp.rh_floor = base->rh_floor; e->lp.epsP = base->rh_floor * base->T_floor; e->lp.gm1 = ph.gm1;
     Tables t;
     build_tables(e->nb, e->q, &t);

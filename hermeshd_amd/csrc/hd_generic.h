@@ -153,6 +153,18 @@ __device__ __forceinline__ void apply_bc(int kind, int dir, int side, const doub
     }
 }
 
+// physical flux of the configured equations at one point
+template <int NQ, int DIR>
+__device__ __forceinline__ void gflux(const Phys& ph, const double* Q, double* F) {
+    if constexpr (NQ >= 11) {
+        if (ph.ivis == 2) {
+            flux_point10<DIR>(ph, Q, F);
+            return;
+        }
+    }
+    flux_point<NQ, DIR>(ph, Q, F, nullptr);
+}
+
 template <int NB, int NF, int NQ>
 __device__ __forceinline__ void load_trace(const double* __restrict__ q, const Grid& g, long cell,
                                            const double (*bf)[10], double (&out)[NF][NQ]) {
@@ -216,12 +228,21 @@ __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q,
 #pragma unroll
     for (int p = 0; p < NF; p++) {
         double FL[NQ], FR[NQ], out[NQ];
-        flux_point<NQ, DIR>(ph, QL[p], FL, nullptr);
-        flux_point<NQ, DIR>(ph, QR[p], FR, nullptr);
+        gflux<NQ, DIR>(ph, QL[p], FL);
+        gflux<NQ, DIR>(ph, QR[p], FR);
 #pragma unroll
         for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
+        if constexpr (NQ >= 11) {
+            // ivis 2: local Lax-Friedrichs dissipation on every field with the
+            // 10-moment freezing speeds (src/flux.f90:243-263)
+            if (ph.ivis == 2) {
+                const double cw = fmax(cfcal10<DIR>(ph, QL[p]), cfcal10<DIR>(ph, QR[p]));
+#pragma unroll
+                for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]) - 0.5 * cw * (QR[p][f] - QL[p][f]);
+            }
+        }
         if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL[p], QR[p], FL, FR, out);
-        if constexpr (RM == RIEMANN_ROE) roe_point<DIR>(ph, QL[p], QR[p], FL, FR, out);
+        if constexpr (RM == RIEMANN_ROE) roe_point<DIR, NQ>(ph, QL[p], QR[p], FL, FR, out);
 #pragma unroll
         for (int f = 0; f < NQ; f++) fl[((long)f * NF + p) * nfaces + fid] = out[f];
     }
@@ -279,9 +300,9 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
             for (int r = 0; r < NB; r++) s += b[r] * m[f][r];
             Qp[f] = s;
         }
-        flux_point<NQ, 0>(ph, Qp, Fx, nullptr);
-        flux_point<NQ, 1>(ph, Qp, Fy, nullptr);
-        flux_point<NQ, 2>(ph, Qp, Fz, nullptr);
+        gflux<NQ, 0>(ph, Qp, Fx);
+        gflux<NQ, 1>(ph, Qp, Fy);
+        gflux<NQ, 2>(ph, Qp, Fz);
         const double w = T->w3[p];
 #pragma unroll
         for (int f = 0; f < NQ; f++) {
@@ -318,6 +339,19 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
     const double dt = *sp.dt;
     const double faci = 1. / (1. + dt * sp.coll);
     bool bad = false;
+    // NaN check of mode 1, SSP combination, store
+    auto emit = [&](int f, int r, double v) {
+        const long o = qoff(g, c, f, r);
+        if (r == 0 && v != v) bad = true;
+        if (sp.combine == 1) v = 0.5 * (qn[o] + v);
+        else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
+        else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
+        qout[o] = v;
+    };
+    // ivis 2: glflux and source of the diagonal second moments, kept for the
+    // trace-coupled implicit update
+    constexpr int ND = NQ >= 11 ? 3 : 1;
+    double dglf[ND][NB], dsrc[ND][NB];
 #pragma unroll
     for (int f = 0; f < NQ; f++) {
         double xm[NF], xp[NF], ym[NF], yp[NF], zm[NF], zp[NF];
@@ -371,6 +405,38 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 #pragma unroll
         for (int r = 0; r < NB; r++) src[r] = 0.;
         if constexpr (NQ >= 11) {
+            if (sp.ivis == 2 && f >= PXX) {
+                // explicit part of the 10-moment relaxation (src/sources.f90:55-73)
+                double s[NB];
+#pragma unroll
+                for (int r = 0; r < NB; r++) s[r] = 0.;
+                for (int p = 0; p < NPG; p++) {
+                    const double* b = T->bint[p];
+                    double dn = 0., mx = 0., my = 0., mz = 0.;
+#pragma unroll
+                    for (int rr = 0; rr < NB; rr++) {
+                        dn += b[rr] * m[RH][rr];
+                        mx += b[rr] * m[MX][rr];
+                        my += b[rr] * m[MY][rr];
+                        mz += b[rr] * m[MZ][rr];
+                    }
+                    const double dni = 1. / dn;
+                    const double vx = mx * dni, vy = my * dni, vz = mz * dni;
+                    const double vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+                    const double colldn = sp.coll * dn;
+                    double val;
+                    if (f == PXX) val = colldn * (2 * vx2 - vy2 - vz2) * (1. / 3.);
+                    else if (f == PYY) val = colldn * (2 * vy2 - vz2 - vx2) * (1. / 3.);
+                    else if (f == PZZ) val = colldn * (2 * vz2 - vx2 - vy2) * (1. / 3.);
+                    else if (f == PXY) val = colldn * vx * vy;
+                    else if (f == PXZ) val = colldn * vx * vz;
+                    else val = colldn * vy * vz;
+#pragma unroll
+                    for (int r = 0; r < NB; r++) s[r] += T->bwint[p][r] * val;
+                }
+#pragma unroll
+                for (int r = 0; r < NB; r++) src[r] = 0.125 * T->cb[r] * s[r];
+            }
             if (sp.ivis == 0 && f >= PXX && sp.coll != 0.) {
                 for (int r = 0; r < NB; r++) {
                     double s = 0.;
@@ -391,20 +457,48 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 #pragma unroll
             for (int r = 0; r < NB; r++) src[r] += sp.bf[0] * m[MX][r] + sp.bf[1] * m[MY][r] + sp.bf[2] * m[MZ][r];
         // ---- update ----
+        if (sp.mode == 1) {
 #pragma unroll
-        for (int r = 0; r < NB; r++) {
-            const long o = qoff(g, c, f, r);
-            if (sp.mode == 1) {
-                qout[o] = -(glf[r] - src[r]);
+            for (int r = 0; r < NB; r++) qout[qoff(g, c, f, r)] = -(glf[r] - src[r]);
+            continue;
+        }
+        if constexpr (NQ >= 11) {
+            if (sp.ivis == 2 && f >= PXX) {
+                if (f <= PZZ) {
+#pragma unroll
+                    for (int r = 0; r < NB; r++) {
+                        dglf[f - PXX][r] = glf[r];
+                        dsrc[f - PXX][r] = src[r];
+                    }
+                    if (f == PZZ) {
+                        // backward-Euler relaxation of the second moments towards
+                        // their trace (src/integrator.f90:181-228)
+#pragma unroll
+                        for (int r = 0; r < NB; r++) {
+                            const double Q_sum = sp.coll * dt * (m[PXX][r] + m[PYY][r] + m[PZZ][r]) * (1. / 3.);
+                            const double glf_sum =
+                                sp.coll * dt * dt * (dglf[0][r] + dglf[1][r] + dglf[2][r]) * (1. / 3.);
+                            const double src_sum =
+                                sp.coll * dt * dt * (dsrc[0][r] + dsrc[1][r] + dsrc[2][r]) * (1. / 3.);
+#pragma unroll
+                            for (int d = 0; d < 3; d++)
+                                emit(PXX + d, r,
+                                     faci * (m[PXX + d][r] - dt * dglf[d][r] + dt * dsrc[d][r] + Q_sum - glf_sum +
+                                             src_sum));
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < NB; r++) emit(f, r, (m[f][r] - dt * glf[r] + dt * src[r]) * faci);
+                }
                 continue;
             }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
             double v = m[f][r] - dt * (glf[r] - src[r]);
             if (sp.ivis == 1 && f >= PXX) v = faci * v;
-            if (r == 0 && v != v) bad = true;
-            if (sp.combine == 1) v = 0.5 * (qn[o] + v);
-            else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
-            else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
-            qout[o] = v;
+            emit(f, r, v);
         }
     }
     if (bad) {

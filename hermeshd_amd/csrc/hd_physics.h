@@ -8,6 +8,8 @@
 //   hllc_point  : flux_hllc         (src/flux.f90:466-647), ibatten=0 (Davis speeds)
 //   roe_point   : flux_roe          (src/ces/10_moment/dg_3D_hydro_lqc_SEAN.f90:1614-1789)
 //   cfcal       : cfcal             (src/flux.f90:1029-1072)
+//   flux_point10, cfcal10 : the ivis = 2 branches of flux_calc_pnts_r and
+//                 cfcal (src/flux.f90:64-66,89-117,134-155,172-191,1038-1044)
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,6 +25,7 @@ struct Phys {
     double gamma, gm1;       // aindex, aindm1
     double P_floor, rh_floor;
     double colvis, c2d3cv, c4d3cv;
+    int ivis;                // 2: full 10-moment equations (table-driven kernels only)
 };
 
 // DIR: 0=x, 1=y, 2=z.  S: stochastic stress xx,yy,zz,xy,xz,yz or nullptr.
@@ -71,6 +74,68 @@ HD_DEV void flux_point(const Phys& ph, const double* Q, double* F, const double*
             F[PXY] = 0.; F[PXZ] = ph.colvis * vx; F[PYZ] = ph.colvis * vy;
         }
     }
+}
+
+// Full 10-moment flux (ivis = 2): the six stress fields hold the second
+// moments E_ij = rho u_i u_j + P_ij.  The momentum flux is E_{i,DIR} with the
+// isotropic 10-moment pressure P_10 = tr(P)/3 swapped for the equation-of-state
+// pressure P_5; the second moments carry the flux of the 10-moment equations,
+// written out per direction as in the reference.
+template <int DIR>
+HD_DEV void flux_point10(const Phys& ph, const double* Q, double* F) {
+    const double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
+    const double P_xx = Q[PXX], P_yy = Q[PYY], P_zz = Q[PZZ], P_xy = Q[PXY], P_xz = Q[PXZ], P_yz = Q[PYZ];
+    const double dni = 1. / dn;
+    const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
+    const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
+    double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
+    if (P < ph.P_floor) P = ph.P_floor;
+    double P_10 = (1. / 3.) * (P_xx + P_yy + P_zz - dni * smsq);
+    if (P_10 < ph.P_floor) P_10 = ph.P_floor;
+    const double P_5 = P;
+    if constexpr (DIR == 0) {
+        F[RH] = M_x;
+        F[MX] = P_xx - P_10 + P_5; F[MY] = P_xy; F[MZ] = P_xz;
+        F[EN] = (Ener + P) * vx + P_xx * vx + P_xy * vy + P_xz * vz;
+        F[PXX] = vx * (3 * P_xx - 2 * vx * M_x);
+        F[PYY] = 2 * vy * (P_xy - vy * M_x) + vx * P_yy;
+        F[PZZ] = 2 * vz * (P_xz - vz * M_x) + vx * P_zz;
+        F[PXY] = 2 * vx * (P_xy - vy * M_x) + vy * P_xx;
+        F[PXZ] = 2 * vx * (P_xz - vz * M_x) + vz * P_xx;
+        F[PYZ] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vy * vz * M_x;
+    } else if constexpr (DIR == 1) {
+        F[RH] = M_y;
+        F[MX] = P_xy; F[MY] = P_yy - P_10 + P_5; F[MZ] = P_yz;
+        F[EN] = (Ener + P) * vy + P_yy * vy + P_xy * vx + P_yz * vz;
+        F[PXX] = 2 * vx * (P_xy - vx * M_y) + vy * P_xx;
+        F[PYY] = vy * (3 * P_yy - 2 * vy * M_y);
+        F[PZZ] = 2 * vz * (P_yz - vz * M_y) + vy * P_zz;
+        F[PXY] = 2 * vy * (P_xy - vx * M_y) + vx * P_yy;
+        F[PXZ] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vx * vz * M_y;
+        F[PYZ] = 2 * vy * (P_yz - vz * M_y) + vz * P_yy;
+    } else {
+        F[RH] = M_z;
+        F[MX] = P_xz; F[MY] = P_yz; F[MZ] = P_zz - P_10 + P_5;
+        F[EN] = (Ener + P) * vz + P_zz * vz + P_xz * vx + P_yz * vy;
+        F[PXX] = 2 * vx * (P_xz - vx * M_z) + vz * P_xx;
+        F[PYY] = 2 * vy * (P_yz - vy * M_z) + vz * P_yy;
+        F[PZZ] = vz * (3 * P_zz - 2 * vz * M_z);
+        F[PXY] = vx * P_yz + vy * P_xz + vz * P_xy - 2 * vx * vy * M_z;
+        F[PXZ] = 2 * vz * (P_xz - vx * M_z) + vx * P_zz;
+        F[PYZ] = 2 * vz * (P_yz - vy * M_z) + vy * P_zz;
+    }
+}
+
+// freezing speed of the 10-moment equations: |u_dir| + sqrt(gamma*P/rho) with
+// P = tr(P_ij)/3 from the second moments (cfcal, ivis = 2)
+template <int DIR>
+HD_DEV double cfcal10(const Phys& ph, const double* Q) {
+    const double dn = Q[RH], dni = 1. / dn;
+    const double vx = Q[MX] * dni, vy = Q[MY] * dni, vz = Q[MZ] * dni;
+    const double P = (1. / 3.) * (Q[PXX] + Q[PYY] + Q[PZZ] - dn * (vx * vx + vy * vy + vz * vz));
+    const double cs = sqrt(ph.gamma * P * dni);
+    const double v = DIR == 0 ? vx : DIR == 1 ? vy : vz;
+    return fabs(v) + cs;
 }
 
 template <int DIR>
@@ -161,7 +226,7 @@ HD_DEV double cfcal(const Phys& ph, const double* Q) {
 // Roe flux (Eulderink & Mellema) for rh..en at one point; when the Roe sound
 // speed squared is not positive, local Lax-Friedrichs with cfcal speeds, as the
 // legacy flux_cal does (dg_3D_hydro_lqc_SEAN.f90:1255-1289).
-template <int DIR>
+template <int DIR, int NQ = 5>
 HD_DEV void roe_point(const Phys& ph, const double* QL, const double* QR, const double* FL,
                       const double* FR, double* out) {
     int ipar, ip1, ip2;
@@ -192,7 +257,12 @@ HD_DEV void roe_point(const Phys& ph, const double* QL, const double* QR, const 
     const double vcsq = vc0 * vc0 + vc1 * vc1 + vc2 * vc2;
     const double asq = gm1 * (hc - 0.5 * vcsq);
     if (!(asq > 0.0)) {
-        const double cw = fmax(cfcal<DIR>(ph, QL), cfcal<DIR>(ph, QR));
+        double cw;
+        if constexpr (NQ >= 11)
+            cw = ph.ivis == 2 ? fmax(cfcal10<DIR>(ph, QL), cfcal10<DIR>(ph, QR))
+                              : fmax(cfcal<DIR>(ph, QL), cfcal<DIR>(ph, QR));
+        else
+            cw = fmax(cfcal<DIR>(ph, QL), cfcal<DIR>(ph, QR));
         const int idx[5] = {RH, MX, MY, MZ, EN};
         for (int e = 0; e < 5; e++) {
             const int f = idx[e];

@@ -44,7 +44,9 @@ extern "C" {
 typedef struct hd_config {
     int32_t nx, ny, nz;          /* cells (one device holds the whole grid) */
     int32_t nbasis, iquad, nQ;   /* (4,2), (8,2) or (10,3); nQ 5 or 11 */
-    int32_t ivis, ieos;          /* ivis 0 explicit, 1 semi-implicit stress relaxation; ieos 1 */
+    int32_t ivis, ieos;          /* ivis 0 explicit, 1 semi-implicit stress relaxation, 2 full
+                                    10-moment equations (nQ 11, no noise, no adjoint or ensemble:
+                                    table-driven kernels); ieos 1 */
     int32_t riemann;             /* HD_RIEMANN_* (input.f90 ihllc; Roe from the legacy solver) */
     int32_t limiter;             /* 1 = positivity limiter on every stage input */
     int32_t integrator;          /* 2 = heun (SSP-RK2), 3 = shu-osher (SSP-RK3) */
@@ -80,7 +82,7 @@ int hd_limit(hd_ctx *ctx, double *q, void *stream);
 int hd_rhs(hd_ctx *ctx, const double *q, double *rhs, void *stream);
 
 /* One forward-Euler stage: limiter on q_in (in place), then
-   q_out = q_in - dt*(glflux - source) (+ stress relaxation when ivis = 1).
+   q_out = q_in - dt*(glflux - source) (+ implicit stress relaxation when ivis = 1, 2).
    dt_dev points to one double in device memory.  Replaces euler_step
    (src/integrator.f90:93-102) and advance_time_level (125-252). */
 int hd_euler_step(hd_ctx *ctx, double *q_in, double *q_out, const double *dt_dev, void *stream);

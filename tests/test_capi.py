@@ -61,6 +61,13 @@ def test_config_validation():
         HDConfig(nx=4, ny=4, nz=4, bc=("periodic", "wall", "periodic", "periodic", "periodic", "periodic"))
     with pytest.raises(ValueError):
         HDConfig(nx=4, ny=4, nz=4, ivis=1, nQ=5)
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, ivis=2, nQ=5)
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, ivis=2, nQ=11, noise=True)
+    with pytest.raises(ValueError):
+        HDConfig(nx=4, ny=4, nz=4, ivis=3, nQ=11)
+    HDConfig(nx=4, ny=4, nz=4, ivis=2, nQ=11)
     c = vortex_3d()
     assert c.shape == (10, 5, 32, 128, 128) and c.cflm == 0.08
     assert abs(1.0 / c.dxi - 10.0 / 128) < 1e-15
