@@ -12,7 +12,7 @@ import math
 from dataclasses import dataclass, field
 from typing import Sequence, Tuple
 
-BC_NAMES = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3}
+BC_NAMES = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3, "inflow": 4}
 RIEMANN_NAMES = {"central": 0, "hllc": 1, "roe": 2}
 INTEGRATOR_NAMES = {"heun": 2, "shu-osher": 3}
 VALID_BASIS = {(4, 2), (8, 2), (10, 3)}
@@ -92,6 +92,8 @@ class HDConfig:
         for d in range(3):
             if (bc[2 * d] == "periodic") != (bc[2 * d + 1] == "periodic"):
                 raise ValueError("a periodic axis must be periodic on both sides")
+        if "inflow" in bc[1:]:
+            raise ValueError("inflow is an x-low boundary (x_bc_inflow, src/boundary.f90:571-592)")
         self.bc = bc
         if min(self.nx, self.ny, self.nz) < 1:
             raise ValueError("grid extents must be positive")

@@ -358,6 +358,7 @@ struct AdjParams {
     const double* qn;
     double* partial;           // per-block partial sums of d(lam.out)/dcoll
     int rhs_only;              // 1: lam_in = (dR/dm)^T lam (no stage map, no limiter, no sums)
+    const double* inflow;      // x-low inflow exterior state (ny,nz,nface,nQ), or nullptr
 };
 
 template <int NB, int NQ>
@@ -378,7 +379,8 @@ template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
 __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, const double* __restrict__ lam,
                                               const Grid& g, const fz::CellPos& cp, const double (&m)[NQ][NB],
                                               const double (&lo)[NQ][NB], double (&rt)[NQ][NB], const Phys& ph,
-                                              double dxi_d, double scale, double faci) {
+                                              double dxi_d, double scale, double faci,
+                                              const double* __restrict__ inflow) {
     constexpr int NF = QP * QP;
     const auto& T = fz::face_tab<NB>();
     const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
@@ -449,6 +451,9 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
                     oth[MX + DIR] = 0.0;
                 }
             }
+            if constexpr (DIR == 0 && SIDE == 0) {
+                if (bc == BC_INFLOW) fz::inflow_ghost<NQ>(inflow, g, cp, p, NF, oth);
+            }
         }
 #pragma unroll
         for (int f = 0; f < NQ; f++) mu[f] *= dxi_d;
@@ -467,6 +472,7 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
                 double gcoef = 1.0;
                 if (bc == BC_NOSLIP && (f == MX || f == MY || f == MZ)) gcoef = 0.0;
                 if (bc == BC_WALL && f == MX + DIR) gcoef = 0.0;
+                if (bc == BC_INFLOW && f <= EN) gcoef = 0.0;   // prescribed: no dependence on the interior
                 if (bc == BC_OUTFLOW) {
                     if (DIR == 0 && SIDE == 1) {
                         if (f == MX && flip_mx) gcoef = -1.0;
@@ -580,21 +586,21 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
         }
     }
     if (parts & 2) {
-        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, ap.inflow);
         fz::fence();
     }
     if (parts & 4) {
-        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, ap.inflow);
         fz::fence();
     }
     if (parts & 8) {
-        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci);
+        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, ap.inflow);
         fz::fence();
     }
     return dcoll;

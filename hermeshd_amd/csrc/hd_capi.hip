@@ -50,10 +50,12 @@ int validate(const hd_config* c) {
     if (c->integrator != 2 && c->integrator != 3) return fail(HD_ERR_INVALID, "integrator must be 2 or 3");
     for (int d = 0; d < 3; d++) {
         for (int s = 0; s < 2; s++)
-            if (c->bc[2 * d + s] < 0 || c->bc[2 * d + s] > 3) return fail(HD_ERR_INVALID, "unknown boundary kind");
+            if (c->bc[2 * d + s] < 0 || c->bc[2 * d + s] > 4) return fail(HD_ERR_INVALID, "unknown boundary kind");
         if ((c->bc[2 * d] == HD_BC_PERIODIC) != (c->bc[2 * d + 1] == HD_BC_PERIODIC))
             return fail(HD_ERR_INVALID, "a periodic axis must be periodic on both sides");
     }
+    for (int s = 1; s < 6; s++)
+        if (c->bc[s] == HD_BC_INFLOW) return fail(HD_ERR_INVALID, "inflow is an x-low boundary");
     if (c->noise && c->nQ != 11) return fail(HD_ERR_INVALID, "noise needs nQ=11 (the stress fields)");
     if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) return fail(HD_ERR_INVALID, "domain lengths must be positive");
     if (!(c->gamma > 1.0)) return fail(HD_ERR_INVALID, "gamma must exceed 1");
@@ -99,6 +101,7 @@ StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int
     for (int d = 0; d < 3; d++) sp.noise[d] = c->noise_f[d];
     sp.noise[3] = c->noise_v;
     sp.nfb = c->nfb;
+    sp.inflow = c->inflow;
     return sp;
 }
 
@@ -127,6 +130,29 @@ int draw_noise(hd_ctx* c, const double* dt_dev, hipStream_t s) {
     return HD_OK;
 }
 
+// immersed solid (apply_cyl_in_2d_pipe_boundaries, src/boundary.f90:686-700,
+// with the mode loop over nbasis as intended): masked cells get density
+// rho_mask with no slope and zero momentum in every mode
+__global__ __launch_bounds__(256) void k_apply_mask(double* __restrict__ q, const unsigned char* __restrict__ mask,
+                                                    long N, int nQ, int nb, double rho_mask) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N || !mask[c]) return;
+    for (int r = 0; r < nb; r++) {
+        q[((long)r * nQ + RH) * N + c] = r == 0 ? rho_mask : 0.0;
+        q[((long)r * nQ + MX) * N + c] = 0.0;
+        q[((long)r * nQ + MY) * N + c] = 0.0;
+        q[((long)r * nQ + MZ) * N + c] = 0.0;
+    }
+}
+
+int apply_mask(hd_ctx* c, double* q, hipStream_t s) {
+    if (!c->mask_on) return HD_OK;
+    hipLaunchKernelGGL(k_apply_mask, dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->mask, c->N, c->nQ, c->nb,
+                       c->rho_mask);
+    HIPCHK(hipGetLastError());
+    return HD_OK;
+}
+
 int advance_step(hd_ctx* c, hipStream_t s) {
     if (!c->cfg.noise) return HD_OK;
     hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, c->step_ctr, 0ull, 1);
@@ -150,6 +176,8 @@ int stage(hd_ctx* c, double* qin, const double* qn, double* qout, const double* 
           bool in_limited = false, bool limit_out = false) {
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
+    if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
+        return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     const bool fuse_lim = c->cfg.limiter && !c->generic;
     if (c->cfg.limiter && !(in_limited && fuse_lim)) o.limiter(c, qin, s);
     StageParams sp = stage_params(c, dt_dev, combine, 0);
@@ -221,6 +249,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         (r = alloc(c, (void**)&c->scratch_dt, sizeof(double))) ||
         (r = alloc(c, (void**)&c->step_ctr, sizeof(unsigned long long))) ||
         (r = alloc(c, (void**)&c->vjp_partial, blocks_for(c->N, 64) * sizeof(double))) ||
+        (cfg->bc[0] == HD_BC_INFLOW &&
+         (r = alloc(c, (void**)&c->inflow, (size_t)g.ny * g.nz * c->nface * c->nQ * sizeof(double)))) ||
         (cfg->integrator == 3 &&
          (r = alloc(c, (void**)&c->stage_buf, (size_t)c->N * c->nQ * c->nb * sizeof(double))))) {
         std::string msg = g_last_error;
@@ -267,7 +297,7 @@ int hd_destroy(hd_ctx* c) {
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {c->dtab,     c->fl[0],    c->fl[1],     c->fl[2],      c->partial,    c->err,
                     c->bad,      c->scratch_dt, c->stage_buf, c->step_ctr, c->noise_f[0], c->noise_f[1],
-                    c->noise_f[2], c->noise_v,  c->vjp_partial};
+                    c->noise_f[2], c->noise_v,  c->vjp_partial, c->inflow, c->mask};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -313,6 +343,8 @@ int hd_rhs(hd_ctx* c, const double* q, double* rhs, void* stream) {
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     hipStream_t s = (hipStream_t)stream;
     if (q == rhs) return fail(HD_ERR_INVALID, "q and rhs must differ");
+    if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
+        return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     // rhs mode does not advance; the scratch scalar (1.0) is the time step of
     // the noise amplitude
     int rc = draw_noise(c, c->scratch_dt, s);
@@ -367,6 +399,7 @@ int hd_update(hd_ctx* c, double* q_io, double* q1, double* q2, const double* dt_
         if ((rc = stage(c, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
         if ((rc = stage(c, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
     }
+    if ((rc = apply_mask(c, q_io, s))) return rc;
     return advance_step(c, s);
 }
 
@@ -384,8 +417,41 @@ int hd_stage(hd_ctx* c, int32_t rk_order, int32_t stage_index, double* q_in, con
     int rc;
     if (stage_index == 1 && (rc = draw_noise(c, dt_dev, s))) return rc;
     if ((rc = stage(c, q_in, combine ? q_n : q_in, q_out, dt_dev, combine, s, false, false))) return rc;
-    if (stage_index == rk_order) return advance_step(c, s);
+    if (stage_index == rk_order) {
+        if ((rc = apply_mask(c, q_out, s))) return rc;
+        return advance_step(c, s);
+    }
     return HD_OK;
+}
+
+int hd_set_inflow(hd_ctx* c, const double* def_dev, void* stream) {
+    if (!c || !def_dev) return fail(HD_ERR_INVALID, "NULL argument");
+    if (c->cfg.bc[0] != HD_BC_INFLOW) return fail(HD_ERR_INVALID, "the x-low boundary is not an inflow");
+    const size_t n = (size_t)c->grid.ny * c->grid.nz * c->nface * c->nQ;
+    HIPCHK(hipMemcpyAsync(c->inflow, def_dev, n * sizeof(double), hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    c->inflow_set = true;
+    return HD_OK;
+}
+
+int hd_set_mask(hd_ctx* c, const uint8_t* mask_dev, double rho_mask, void* stream) {
+    if (!c) return fail(HD_ERR_INVALID, "NULL context");
+    if (!mask_dev) {
+        c->mask_on = false;
+        return HD_OK;
+    }
+    if (!c->mask) {
+        int r = alloc(c, (void**)&c->mask, (size_t)c->N);
+        if (r) return r;
+    }
+    HIPCHK(hipMemcpyAsync(c->mask, mask_dev, (size_t)c->N, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    c->rho_mask = rho_mask;
+    c->mask_on = true;
+    return HD_OK;
+}
+
+int hd_apply_mask(hd_ctx* c, double* q, void* stream) {
+    if (!c || !q) return fail(HD_ERR_INVALID, "NULL argument");
+    return apply_mask(c, q, (hipStream_t)stream);
 }
 
 int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* lam_in, const double* dt_dev,
@@ -397,6 +463,9 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
     if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
+    if (c->mask_on) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the immersed-solid mask");
+    if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
+        return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     adj::AdjParams ap;
     ap.dt = dt_dev;
     ap.coll = c->cfg.coll;
@@ -413,6 +482,7 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     if (ap.out && ap.a != 0. && !ap.qn) return fail(HD_ERR_INVALID, "qn is required with out when a != 0");
     ap.partial = args->dcoll_dev ? c->vjp_partial : nullptr;
     ap.rhs_only = 0;
+    ap.inflow = c->inflow;
     hipStream_t s = (hipStream_t)stream;
     o.stage_vjp(c, q_in, lam_out, lam_in, ap, s);
     HIPCHK(hipGetLastError());
@@ -432,6 +502,8 @@ int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void*
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
     if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
+    if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
+        return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     adj::AdjParams ap{};
     ap.dt = c->scratch_dt;
     ap.coll = c->cfg.coll;
@@ -441,6 +513,7 @@ int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void*
     for (int d = 0; d < 3; d++) ap.bf[d] = c->cfg.body_force[d];
     ap.lp = c->lp;
     ap.rhs_only = 1;
+    ap.inflow = c->inflow;
     o.stage_vjp(c, q, lam, out, ap, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return HD_OK;

@@ -40,6 +40,13 @@ struct hd_ctx {
     double* noise_f[3] = {nullptr, nullptr, nullptr};
     double* noise_v = nullptr;
     long nfb = 0;                  // face lattice size (nx+1)(ny+1)(nz+1)
+    // x-low inflow exterior state (ny,nz,nface,nQ), set by hd_set_inflow
+    double* inflow = nullptr;
+    bool inflow_set = false;
+    // immersed-solid mask (hd_set_mask), applied after every step
+    unsigned char* mask = nullptr;
+    bool mask_on = false;
+    double rho_mask = 0.;
     double kap_f = 0., kap_v = 0.; // point amplitude factors of the face and volume rules
     bool generic = false;          // HD_KERNELS=generic selects the table-driven kernels
     int64_t bytes = 0;

@@ -112,6 +112,7 @@ StageParams ens_params(const hd_ens* e, const double* dt_dev, int combine, int m
     sp.lp = e->lp;
     for (int d = 0; d < 4; d++) sp.noise[d] = nullptr;  // set per realization by the kernel
     sp.nfb = 0;
+    sp.inflow = nullptr;
     return sp;
 }
 
@@ -182,6 +183,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     int rc = validate(base);
     if (rc) return rc;
     if (base->ivis == 2) return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (ivis 0 or 1)");
+    if (base->bc[0] == HD_BC_INFLOW) return fail(HD_ERR_UNSUPPORTED, "ensembles take no inflow boundary");
     for (int r = 0; r < nreal; r++) {
         hd_config c = *base;
         c.nx = dims[3 * r]; c.ny = dims[3 * r + 1]; c.nz = dims[3 * r + 2];

@@ -202,6 +202,21 @@ __device__ __forceinline__ void load_modes(const double* __restrict__ q, const G
         for (int f = 0; f < NQ; f++) m[f][r] = q[((long)r * NQ + f) * g.N + c];
 }
 
+// Exterior state of an x-low inflow face point: inflow holds the reference's
+// Qxlo_ext_def(ny,nz,nface,nQ) in Fortran order.
+template <int NQ>
+__device__ __forceinline__ void inflow_ghost(const double* __restrict__ inflow, const Grid& g, const CellPos& cp,
+                                             int p, int nface, double* oth) {
+    const long plane = (long)g.ny * g.nz;
+    const long base = (long)cp.j + (long)g.ny * cp.k + plane * p;
+    const long fs = plane * nface;  // stride of one field
+    oth[RH] = inflow[base + fs * RH];
+    oth[MX] = inflow[base + fs * MX];
+    oth[MY] = 0.0;
+    oth[MZ] = 0.0;
+    oth[EN] = inflow[base + fs * EN];
+}
+
 // One face of the cell: DIR 0/1/2, SIDE 0 (minus) / 1 (plus).
 template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE>
 __device__ __forceinline__ void face_term(const double* __restrict__ q, const Grid& g, const CellPos& cp,
@@ -275,6 +290,11 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
                 } else if (zero_normal) {
                     oth[MX + DIR] = 0.0;
                 }
+            }
+            if constexpr (DIR == 0 && SIDE == 0) {
+                // x_bc_inflow (src/boundary.f90:571-592): prescribed rh, mx, en;
+                // no transverse momentum; the stress fields from the interior
+                if (bc == BC_INFLOW) inflow_ghost<NQ>(sp.inflow, g, cp, p, QP * QP, oth);
             }
         }
         const double* QL = SIDE ? own : oth;

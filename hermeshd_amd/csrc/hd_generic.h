@@ -117,13 +117,24 @@ __global__ __launch_bounds__(256) void k_limiter(double* __restrict__ q, const T
 // ---------------------------------------------------------------------------
 template <int NF, int NQ>
 __device__ __forceinline__ void apply_bc(int kind, int dir, int side, const double (&in)[NF][NQ],
-                                         double (&ex)[NF][NQ]) {
+                                         double (&ex)[NF][NQ], const double* inflow = nullptr, long jk = 0,
+                                         long plane = 0) {
     const int normal = MX + dir;
 #pragma unroll
     for (int p = 0; p < NF; p++)
 #pragma unroll
         for (int f = 0; f < NQ; f++) ex[p][f] = in[p][f];
-    if (kind == BC_NOSLIP) {
+    if (kind == BC_INFLOW) {
+        // x_bc_inflow (src/boundary.f90:571-592); inflow is (ny,nz,nface,nQ), jk = j + ny*k
+#pragma unroll
+        for (int p = 0; p < NF; p++) {
+            ex[p][RH] = inflow[jk + plane * (p + (long)NF * RH)];
+            ex[p][MX] = inflow[jk + plane * (p + (long)NF * MX)];
+            ex[p][MY] = 0.0;
+            ex[p][MZ] = 0.0;
+            ex[p][EN] = inflow[jk + plane * (p + (long)NF * EN)];
+        }
+    } else if (kind == BC_NOSLIP) {
 #pragma unroll
         for (int p = 0; p < NF; p++) ex[p][MX] = ex[p][MY] = ex[p][MZ] = 0.0;
     } else if (kind == BC_WALL) {
@@ -190,7 +201,8 @@ __device__ __forceinline__ void load_trace(const double* __restrict__ q, const G
 // ---------------------------------------------------------------------------
 template <int NB, int QP, int NQ, int RM, int DIR>
 __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q, double* __restrict__ fl,
-                                                   const Tables* __restrict__ T, Grid g, Phys ph) {
+                                                   const Tables* __restrict__ T, Grid g, Phys ph,
+                                                   const double* __restrict__ inflow) {
     constexpr int NF = QP * QP;
     const int ni = g.nx + (DIR == 0), nj = g.ny + (DIR == 1), nk = g.nz + (DIR == 2);
     const long nfaces = (long)ni * nj * nk;
@@ -214,7 +226,7 @@ __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q,
             const int ii = DIR == 0 ? g.nx - 1 : i, jj = DIR == 1 ? g.ny - 1 : j, kk = DIR == 2 ? g.nz - 1 : k;
             load_trace<NB, NF, NQ>(q, g, cell_at(ii, jj, kk), bp, QL);
         } else {
-            apply_bc<NF, NQ>(g.bc[2 * DIR], DIR, 0, QR, QL);
+            apply_bc<NF, NQ>(g.bc[2 * DIR], DIR, 0, QR, QL, inflow, (long)j + (long)g.ny * k, (long)g.ny * g.nz);
         }
     }
     if (a == n_ax) {
@@ -261,6 +273,7 @@ struct StageParams {
     LimParams lp;
     const double* noise[4];  // stochastic stress of the step: x, y, z face families, volume points
     long nfb;                // face lattice size (nx+1)(ny+1)(nz+1)
+    const double* inflow;    // x-low inflow exterior state (ny,nz,nface,nQ), Fortran order
 };
 
 // ---------------------------------------------------------------------------

@@ -16,9 +16,9 @@ struct Impl {
     static void faces(hd_ctx* c, const double* q, hipStream_t s) {
         const Grid& g = c->grid;
         long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz, nfz = (long)g.nx * g.ny * (g.nz + 1);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph);
-        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph, c->inflow);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph, c->inflow);
+        hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph, c->inflow);
     }
     static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
         hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],

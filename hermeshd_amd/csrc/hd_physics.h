@@ -18,7 +18,7 @@
 namespace hd {
 
 enum : int { RH = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PXX = 5, PYY = 6, PZZ = 7, PXY = 8, PXZ = 9, PYZ = 10 };
-enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2, BC_NOSLIP = 3 };
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2, BC_NOSLIP = 3, BC_INFLOW = 4 };
 enum : int { RIEMANN_CENTRAL = 0, RIEMANN_HLLC = 1, RIEMANN_ROE = 2 };
 
 struct Phys {

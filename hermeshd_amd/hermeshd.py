@@ -32,25 +32,31 @@ import torch
 from ._lib import NonFiniteError  # noqa: F401  (re-exported)
 from .checkpoint import read_checkpoint, write_checkpoint
 from .config import HDConfig
-from .ics import set_ic
+from .ics import setup_problem
 from .solver import Solver
 
 _input = {"cfg": None, "icid": None, "tf": None, "ntout": 200, "clt": 2.0, "check_every": 1,
-          "iread": 0, "iwrite": 0, "fpre": "Qout", "data_dir": "data", "resuming": True, "mpi": (1, 1, 1)}
+          "iread": 0, "iwrite": 0, "fpre": "Qout", "data_dir": "data", "resuming": True, "mpi": (1, 1, 1),
+          "vis": None, "cylinder": False, "inflow": None}
 _state = {"solver": None, "q1_dev": None, "steps": 0}
 
 
 def configure(cfg: HDConfig, icid: int | None = None, tf: float | None = None, ntout: int = 200,
               clt: float = 2.0, check_every: int = 1, iread: int = 0, iwrite: int = 0, fpre: str = "Qout",
-              data_dir: str = "data", resuming: bool = True, mpi=(1, 1, 1)) -> None:
+              data_dir: str = "data", resuming: bool = True, mpi=(1, 1, 1), vis: float | None = None,
+              cylinder: bool = False, inflow=None) -> None:
     """Set the run's switches (the reference's input.f90 module).  iread > 0
     starts from checkpoint dump iread (keeping its t, dt, nout when resuming);
     iwrite = 1 writes a checkpoint at every output time, alternating dumps 1
     and 2, split over `mpi` = (mpi_nx, mpi_ny, mpi_nz) rank files as the
-    reference would (src/hermeshd.f90:206-266, src/initialize.f90:394-565)."""
+    reference would (src/hermeshd.f90:206-266, src/initialize.f90:394-565).
+    vis: the viscosity from which icid 0, 5, 6, 7 set coll and colvis (as
+    set_ic does); cylinder: apply the cylinder mask of icid 6, 7 after every
+    step; inflow: the x-low inflow state (nQ, nface, nz, ny) when cfg.bc[0] is
+    "inflow" and the initial condition does not define one."""
     _input.update(cfg=cfg, icid=icid, tf=tf, ntout=ntout, clt=clt, check_every=max(0, int(check_every)),
                   iread=int(iread), iwrite=int(iwrite), fpre=fpre, data_dir=data_dir, resuming=bool(resuming),
-                  mpi=tuple(mpi))
+                  mpi=tuple(mpi), vis=vis, cylinder=bool(cylinder), inflow=inflow)
 
 
 def _cfg() -> HDConfig:
@@ -98,9 +104,21 @@ class Hermeshd:
         """setup (src/hermeshd.f90:67-114): tables and context, initial
         condition (when icid is configured), integrator and boundaries."""
         cfg = _cfg()
+        q_ic = inflow = mask = None
+        if _input["icid"] is not None and not _input["iread"]:
+            cfg, q_ic, inflow, mask = setup_problem(cfg, _input["icid"], _input["vis"])
+            _input["cfg"] = cfg
+        if _input["inflow"] is not None:
+            inflow = _input["inflow"]
         if _state["solver"] is not None:
             _state["solver"].close()
         _state["solver"] = Solver(cfg)
+        if cfg.bc[0] == "inflow":
+            if inflow is None:
+                raise ValueError("the x-low inflow boundary needs a state: configure(..., inflow=...) or icid 5-7")
+            _state["solver"].set_inflow(inflow)
+        if _input["cylinder"] and mask is not None:
+            _state["solver"].set_mask(mask)
         _state["steps"] = 0
         _set_scalar(t, 0.0)
         # init_temporal_params (src/initialize.f90:332-345)
@@ -120,8 +138,8 @@ class Hermeshd:
                 _set_scalar(t, tp)
                 _set_scalar(dt, dtp)
                 _set_scalar(nout, noutp)
-        elif _input["icid"] is not None:
-            q0 = torch.from_numpy(set_ic(cfg, _input["icid"]))
+        elif q_ic is not None:
+            q0 = torch.from_numpy(q_ic)
         if q0 is not None:
             if isinstance(q_io, torch.Tensor):
                 q_io.copy_(q0.to(q_io.device))

@@ -152,11 +152,98 @@ def taylor_green(cfg: HDConfig, rho: float = 1.0, noise: float = 1e-4, seed: int
     return q
 
 
+def hydro_jet(cfg: HDConfig, seed: int = 1234) -> np.ndarray:
+    """icid 0, the 2D unstable jet (src/initialcon.f90:79-114): rho = 1,
+    p = T_floor, x momentum 1/cosh(20 y/ly_u) and a y momentum perturbation
+    0.005*(U - 1/2)/cosh^2(20 y/ly_u).  The reference draws U with Fortran's
+    unseeded random_number; here it is numpy PCG64 with the given seed."""
+    q = background(cfg)
+    _, yc, _ = cell_centres(cfg)
+    shp = (cfg.nz, cfg.ny, cfg.nx)
+    dn, te = 1.0, cfg.T_floor
+    pr = te * dn
+    beta, beta2 = 1.0, 0.005
+    lyu = cfg.ly / 2.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rnum = rng.uniform(0.0, 1.0, size=shp) - 0.5
+    ch = np.cosh(20 * yc / lyu)[None, :, None]
+    smx = np.broadcast_to(beta * dn / ch, shp)
+    smy = beta2 * rnum / ch ** 2
+    q[0, RH] = dn
+    q[0, MX] = smx
+    q[0, MY] = smy
+    q[0, MZ] = 0.0
+    q[0, EN] = pr / (cfg.gamma - 1.0) + 0.5 * (smx ** 2 + smy ** 2) / dn
+    return q
+
+
+def inflow_profile(cfg: HDConfig, ux_amb: float, den: float, pres: float) -> np.ndarray:
+    """Parabolic x-low inflow state of set_xlobc_inflow
+    (src/boundary.f90:595-622): u = 4 ux_amb y'(ly - y')/ly^2 with y' the cell
+    centre's height above the bottom wall, the same at every face point.
+    Returns (nQ, nface, nz, ny), the layout of the reference's
+    Qxlo_ext_def(ny,nz,nface,nQ); fields other than rh, mx, en are 0."""
+    _, yc, _ = cell_centres(cfg)
+    yp = yc - (-(cfg.ly / 2.0))
+    vx = 4 * ux_amb * yp * (cfg.ly - yp) / cfg.ly ** 2
+    nf = cfg.iquad ** 2
+    d = np.zeros((cfg.nQ, nf, cfg.nz, cfg.ny))
+    d[RH] = den
+    d[MX] = (den * vx)[None, None, :]
+    d[EN] = (pres / (cfg.gamma - 1.0) + 0.5 * den * vx ** 2)[None, None, :]
+    return d
+
+
+def cylinder_mask(cfg: HDConfig, xctr: float, yctr: float, rad: float) -> np.ndarray:
+    """Cells whose centre lies in the disc of radius rad about (xctr, yctr),
+    in every z plane (cyl_in_2d_pipe_mask, src/boundary.f90:625-648).
+    Returns (nz, ny, nx) bool."""
+    xc, yc, _ = cell_centres(cfg)
+    r2 = (xc[None, :] - xctr) ** 2 + (yc[:, None] - yctr) ** 2
+    return np.broadcast_to(r2 <= rad ** 2, (cfg.nz, cfg.ny, cfg.nx)).copy()
+
+
+def poiseuille_flow(cfg: HDConfig):
+    """icid 5 (src/initialcon.f90:244-285): fluid at rest, rho = 1,
+    p = T_floor, with the parabolic inflow of peak 0.3.  Returns (q, inflow)."""
+    dn, pr = 1.0, cfg.T_floor
+    return uniform(cfg, rho=dn, T=pr / dn), inflow_profile(cfg, 3.0e-1, dn, pr)
+
+
+def pipe_cylinder_2d(cfg: HDConfig, version: int = 0):
+    """icid 6 (version 0, peak inflow 0.3) / icid 7 (version 1, 1.5)
+    (src/initialcon.f90:289-361): fluid at rest, rho = 1, p = T_floor, the
+    parabolic inflow, and a cylinder of radius ly/8 centred ly/2 from the left
+    and bottom walls.  Returns (q, inflow, mask)."""
+    dn, pr = 1.0, cfg.T_floor
+    ux_amb = 0.3 if version == 0 else 1.5
+    lxd, lyd = -(cfg.lx / 2.0), -(cfg.ly / 2.0)
+    mask = cylinder_mask(cfg, lxd + cfg.ly / 2.0, lyd + cfg.ly / 2.0, cfg.ly / 8.0)
+    return uniform(cfg, rho=dn, T=pr / dn), inflow_profile(cfg, ux_amb, dn, pr), mask
+
+
+def reference_relaxation(cfg: HDConfig, icid: int, vis: float | None = None) -> HDConfig:
+    """coll and colvis as set_ic leaves them (src/initialcon.f90:20-72): zero
+    for ivis 0, then icid 0, 5, 6 and 7 set coll = rho*T/vis with rho = 1,
+    T = T_floor, and colvis = coll*vis (for any ivis)."""
+    if icid in (0, 5, 6, 7):
+        if vis is None:
+            raise ValueError(f"icid {icid} sets coll from the viscosity: give vis")
+        coll = 1.0 * cfg.T_floor / vis
+        return cfg.replace(coll=coll, colvis=coll * vis)
+    if cfg.ivis == 0:
+        return cfg.replace(coll=0.0, colvis=0.0)
+    return cfg
+
+
 def set_ic(cfg: HDConfig, icid: int) -> np.ndarray:
-    """The reference's icid switch for the generators this package carries
-    (src/initialcon.f90:41-69): 1/2 vortex v0/v1, 3/4 Sod v1/v2 (with the
-    reference's p = P_base scaling replaced by the explicit states of
-    ``sod``)."""
+    """The reference's icid switch (src/initialcon.f90:41-69): 0 jet, 1/2
+    vortex v0/v1, 3/4 Sod v1/v2 (with the reference's p = P_base scaling
+    replaced by the explicit states of ``sod``), 5 Poiseuille, 6/7 cylinder
+    in a pipe.  Returns the state; ``setup_problem`` also returns the inflow
+    state and the cylinder mask of 5-7 and the relaxation parameters."""
+    if icid == 0:
+        return hydro_jet(cfg)
     if icid == 1:
         return isentropic_vortex(cfg, ver=0)
     if icid == 2:
@@ -165,4 +252,23 @@ def set_ic(cfg: HDConfig, icid: int) -> np.ndarray:
         return sod(cfg, ver=0)
     if icid == 4:
         return sod(cfg, ver=1)
-    raise ValueError(f"icid {icid} is not provided (supported: 1, 2, 3, 4)")
+    if icid == 5:
+        return poiseuille_flow(cfg)[0]
+    if icid in (6, 7):
+        return pipe_cylinder_2d(cfg, icid - 6)[0]
+    raise ValueError(f"icid {icid} is not one of the reference's 0-7")
+
+
+def setup_problem(cfg: HDConfig, icid: int, vis: float | None = None):
+    """Everything set_ic sets for icid: (cfg, state, x-low inflow state or
+    None, cylinder mask or None).  With vis given, cfg takes the reference's
+    coll and colvis (``reference_relaxation``); with vis None it is kept."""
+    cfg2 = cfg if vis is None else reference_relaxation(cfg, icid, vis)
+    inflow = mask = None
+    if icid == 5:
+        q, inflow = poiseuille_flow(cfg2)
+    elif icid in (6, 7):
+        q, inflow, mask = pipe_cylinder_2d(cfg2, icid - 6)
+    else:
+        q = set_ic(cfg2, icid)
+    return cfg2, q, inflow, mask

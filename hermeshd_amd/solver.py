@@ -147,6 +147,34 @@ class Solver:
         """Step index that keys the stochastic stress (advanced by every update)."""
         _lib.check(_lib.lib().hd_set_step(self._h, int(step), _stream()))
 
+    def set_inflow(self, inflow_def) -> None:
+        """Exterior state of the x-low inflow boundary: (nQ, nface, nz, ny)
+        float64 (numpy or torch), the layout of the reference's
+        Qxlo_ext_def(ny,nz,nface,nQ)."""
+        nf = self.cfg.iquad ** 2
+        want = (self.cfg.nQ, nf, self.cfg.nz, self.cfg.ny)
+        t = torch.as_tensor(inflow_def, dtype=torch.float64).to(self.device).contiguous()
+        if tuple(t.shape) != want:
+            raise ValueError(f"inflow state must have shape {want}")
+        self._inflow = t
+        _lib.check(_lib.lib().hd_set_inflow(self._h, C.c_void_p(t.data_ptr()), _stream()))
+
+    def set_mask(self, mask, rho_mask: float = 1.25) -> None:
+        """Immersed-solid cells, (nz, ny, nx) bool (numpy or torch); None
+        switches the mask off.  Applied after every step."""
+        if mask is None:
+            self._mask = None
+            _lib.check(_lib.lib().hd_set_mask(self._h, None, 0.0, _stream()))
+            return
+        t = torch.as_tensor(mask).to(torch.uint8).to(self.device).contiguous()
+        if tuple(t.shape) != (self.cfg.nz, self.cfg.ny, self.cfg.nx):
+            raise ValueError("mask must have shape (nz, ny, nx)")
+        self._mask = t
+        _lib.check(_lib.lib().hd_set_mask(self._h, C.c_void_p(t.data_ptr()), float(rho_mask), _stream()))
+
+    def apply_mask(self, q: torch.Tensor) -> None:
+        _lib.check(_lib.lib().hd_apply_mask(self._h, self._ptr(q), _stream()))
+
     def check(self) -> None:
         """Synchronise and raise NonFiniteError if a stage produced NaN."""
         bad = C.c_int64(-1)

@@ -35,6 +35,7 @@ extern "C" {
 #define HD_BC_OUTFLOW 1
 #define HD_BC_WALL 2
 #define HD_BC_NOSLIP 3
+#define HD_BC_INFLOW 4   /* x-low only: prescribed rh, mx, en (x_bc_inflow, src/boundary.f90:571-592) */
 /* face solvers for rh..en; other fields always use the central flux */
 #define HD_RIEMANN_CENTRAL 0
 #define HD_RIEMANN_HLLC 1
@@ -148,6 +149,27 @@ int hd_stage_vjp(hd_ctx *ctx, const double *q_in, const double *lam_out, double 
    so that a restarted run draws the same noise.  Replaces the stream set-up
    of random_init (src/random.f90:36-39). */
 int hd_set_step(hd_ctx *ctx, uint64_t step, void *stream);
+
+/* Exterior state of an HD_BC_INFLOW x-low boundary: def_dev points to the
+   reference's Qxlo_ext_def(ny,nz,nface,nQ) (Fortran order, device memory),
+   copied into the context on `stream`; rh, mx and en are used, the transverse
+   momenta are 0 and the stress fields come from the interior.  Required
+   before the first stage of an inflow configuration.  Replaces the array that
+   set_xlobc_inflow fills (src/boundary.f90:595-622) for x_bc_inflow
+   (571-592). */
+int hd_set_inflow(hd_ctx *ctx, const double *def_dev, void *stream);
+
+/* Immersed-solid mask: mask_dev is (nx,ny,nz) uint8 in Fortran order
+   (1 = solid), copied into the context; NULL switches the mask off.  With a
+   mask, hd_update, hd_step and the last stage of hd_stage reset the solid
+   cells after the step: density rho_mask with no slope, zero momentum in
+   every mode (apply_cyl_in_2d_pipe_boundaries, src/boundary.f90:686-700,
+   and the mask of cyl_in_2d_pipe_mask, 625-648).  The adjoint entry points
+   refuse a masked context. */
+int hd_set_mask(hd_ctx *ctx, const uint8_t *mask_dev, double rho_mask, void *stream);
+
+/* Applies the mask to q now (no-op without a mask). */
+int hd_apply_mask(hd_ctx *ctx, double *q, void *stream);
 
 /* Waits for the context's work on `stream` and reports the NaN flag raised by
    any stage since the last check (HD_ERR_NONFINITE, with the first bad cell

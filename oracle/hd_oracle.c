@@ -70,6 +70,9 @@ struct orc_ctx {
     /* face fluxes flux_x(nface,nx+1,ny,nz,nQ) etc. (src/flux.f90:11-13) */
     double *fx, *fy, *fz;
     double *glf, *src, *integ;
+    double *inflow;         /* x-low inflow exterior state (ny,nz,nface,nQ), or NULL */
+    uint8_t *mask;          /* immersed-solid cells, or NULL */
+    double rho_mask;
 };
 
 static inline long qidx(const orc_ctx *c, int i, int j, int k, int f, int r) {
@@ -175,6 +178,10 @@ orc_ctx *orc_create(const orc_config *cfg) {
     if (cfg->ivis >= 1 && cfg->nQ != 11) { set_err(-1, "ivis=1,2 need nQ=11"); return NULL; }
     if (cfg->ivis == 2 && cfg->noise) { set_err(-1, "ivis=2 has no stochastic stress"); return NULL; }
     if (cfg->noise && cfg->nQ != 11) { set_err(-1, "noise needs nQ=11"); return NULL; }
+    for (int s = 0; s < 6; s++) {
+        if (cfg->bc[s] < 0 || cfg->bc[s] > ORC_BC_INFLOW) { set_err(-1, "unknown boundary kind"); return NULL; }
+        if (cfg->bc[s] == ORC_BC_INFLOW && s != 0) { set_err(-1, "inflow is an x-low boundary"); return NULL; }
+    }
     for (int d = 0; d < 3; d++) {
         int lo = cfg->bc[2 * d], hi = cfg->bc[2 * d + 1];
         if ((lo == ORC_BC_PERIODIC) != (hi == ORC_BC_PERIODIC)) {
@@ -220,7 +227,46 @@ void orc_destroy(orc_ctx *c) {
     if (!c) return;
     for (int s = 0; s < 6; s++) { free(c->ext[s]); free(c->intr[s]); }
     free(c->fx); free(c->fy); free(c->fz); free(c->glf); free(c->src); free(c->integ);
+    free(c->inflow); free(c->mask);
     free(c);
+}
+
+int orc_set_inflow(orc_ctx *c, const double *def) {
+    size_t n = (size_t)c->c.ny * c->c.nz * c->nface * c->nQ;
+    if (!c->inflow) c->inflow = (double *)malloc(n * sizeof(double));
+    memcpy(c->inflow, def, n * sizeof(double));
+    return 0;
+}
+
+int orc_set_mask(orc_ctx *c, const uint8_t *mask, double rho_mask) {
+    free(c->mask);
+    c->mask = NULL;
+    if (!mask) return 0;
+    c->mask = (uint8_t *)malloc((size_t)c->ncell);
+    memcpy(c->mask, mask, (size_t)c->ncell);
+    c->rho_mask = rho_mask;
+    return 0;
+}
+
+/* immersed solid (apply_cyl_in_2d_pipe_boundaries, src/boundary.f90:686-700,
+   with its intent: the mode loop there runs over nQ): in masked cells the
+   density is rho_mask (mode 1) with no slope, and every momentum mode is 0 */
+int orc_apply_mask(orc_ctx *c, double *q) {
+    if (!c->mask) return 0;
+    const orc_config *g = &c->c;
+    for (int k = 0; k < g->nz; k++)
+        for (int j = 0; j < g->ny; j++)
+            for (int i = 0; i < g->nx; i++) {
+                long e = i + (long)g->nx * (j + (long)g->ny * k);
+                if (!c->mask[e]) continue;
+                for (int r = 0; r < c->nb; r++) {
+                    q[qidx(c, i, j, k, RH, r)] = r == 0 ? c->rho_mask : 0.0;
+                    q[qidx(c, i, j, k, MX, r)] = 0.0;
+                    q[qidx(c, i, j, k, MY, r)] = 0.0;
+                    q[qidx(c, i, j, k, MZ, r)] = 0.0;
+                }
+            }
+    return 0;
 }
 
 int orc_set_step_index(orc_ctx *c, uint64_t step) { c->step_index = step; return 0; }
@@ -719,7 +765,17 @@ static void exchange_and_bc(orc_ctx *c, const double *q) {
             for (int a = 0; a < na[s]; a++) {
 #define EX(p, f) c->ext[s][BIDX(na[s], nbb[s], a, b, p, f)]
 #define IN(p, f) c->intr[s][BIDX(na[s], nbb[s], a, b, p, f)]
-                if (bc == ORC_BC_NOSLIP) {
+                if (bc == ORC_BC_INFLOW) {
+                    /* x_bc_inflow (src/boundary.f90:571-592): interior copy, then
+                       rh, mx, en from the prescribed state and no transverse momentum */
+                    for (int p = 0; p < nface; p++) {
+                        EX(p, RH) = c->inflow[BIDX(na[s], nbb[s], a, b, p, RH)];
+                        EX(p, MX) = c->inflow[BIDX(na[s], nbb[s], a, b, p, MX)];
+                        EX(p, MY) = 0.0;
+                        EX(p, MZ) = 0.0;
+                        EX(p, EN) = c->inflow[BIDX(na[s], nbb[s], a, b, p, EN)];
+                    }
+                } else if (bc == ORC_BC_NOSLIP) {
                     for (int p = 0; p < nface; p++) EX(p, MX) = EX(p, MY) = EX(p, MZ) = 0.0;
                 } else if (bc == ORC_BC_WALL) {
                     for (int p = 0; p < nface; p++) EX(p, normal) = 0.0;
@@ -951,6 +1007,7 @@ int orc_glflux(orc_ctx *c, const double *q, double *glflux) {
     double ae, az, aev, azv;
     noise_amps(c, c->noise_dt, 0, &ae, &az);
     noise_amps(c, c->noise_dt, 1, &aev, &azv);
+    if (g->bc[0] == ORC_BC_INFLOW && !c->inflow) return set_err(-1, "inflow boundary without its state (orc_set_inflow)");
     exchange_and_bc(c, q);
     calc_face_fluxes(c, q, ae, az);
     inner_integral(c, q, aev, azv);
@@ -1030,7 +1087,8 @@ int orc_source(orc_ctx *c, const double *q, double *source) {
 
 int orc_rhs(orc_ctx *c, const double *q, double *rhs) {
     long n = c->ncell * c->nQ * c->nb;
-    orc_glflux(c, q, c->glf);
+    int rc = orc_glflux(c, q, c->glf);
+    if (rc) return rc;
     orc_source(c, q, c->src);
     for (long e = 0; e < n; e++) rhs[e] = -(c->glf[e] - c->src[e]);
     return 0;
@@ -1043,8 +1101,9 @@ int orc_euler_step(orc_ctx *c, double *q_in, double *q_out, double dt) {
     if (g->limiter) orc_limiter(c, q_in);
     const double keep_dt = c->noise_dt; /* the stage's dt sets the noise amplitude */
     c->noise_dt = dt;
-    orc_glflux(c, q_in, c->glf);
+    int rc = orc_glflux(c, q_in, c->glf);
     c->noise_dt = keep_dt;
+    if (rc) return rc;
     orc_source(c, q_in, c->src);
     double faci = 1. / (1. + dt * g->coll);
     long n = c->ncell;
@@ -1118,6 +1177,7 @@ int orc_update(orc_ctx *c, double *q_io, double *q1, double *q2, double dt) {
         if ((rc = orc_euler_step(c, q1, q2, dt))) return rc;
         for (long e = 0; e < n; e++) q_io[e] = 0.5 * (q_io[e] + q2[e]);
     }
+    orc_apply_mask(c, q_io);
     c->step_index++; /* the noise of the next step is a fresh draw */
     return 0;
 }

@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_WALL = 2, ORC_BC_NOSLIP = 3 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_WALL = 2, ORC_BC_NOSLIP = 3, ORC_BC_INFLOW = 4 };
 enum { ORC_FLUX_CENTRAL = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_ROE = 2 };
 
 typedef struct orc_config {
@@ -46,6 +46,13 @@ orc_ctx *orc_create(const orc_config *cfg);
 void orc_destroy(orc_ctx *c);
 int orc_set_step_index(orc_ctx *c, uint64_t step);    /* noise counter; orc_update advances it by one */
 int orc_set_noise_dt(orc_ctx *c, double dt);          /* dt of the noise amplitude in orc_rhs (default 1) */
+/* prescribed exterior state of an x-low inflow boundary: def(ny,nz,nface,nQ)
+   in Fortran order (Qxlo_ext_def); rh, mx and en are read */
+int orc_set_inflow(orc_ctx *c, const double *def);
+/* immersed-solid mask (nx,ny,nz) in Fortran order, 1 = solid; NULL clears.
+   orc_update applies it after every step (orc_apply_mask) */
+int orc_set_mask(orc_ctx *c, const uint8_t *mask, double rho_mask);
+int orc_apply_mask(orc_ctx *c, double *q);
 
 /* tables (for pinning tests); arrays sized by the caller */
 int orc_tables(const orc_ctx *c, double *xq, double *w1, double *w2, double *w3,

@@ -72,6 +72,9 @@ def lib():
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, dp]
         L.orc_noise_stress.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double, dp]
+        L.orc_set_inflow.argtypes = [C.c_void_p, dp]
+        L.orc_set_mask.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_apply_mask.argtypes = [C.c_void_p, dp]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_bad_cell.restype = C.c_int64
         _lib = L
@@ -149,6 +152,24 @@ class Oracle:
 
     def set_noise_dt(self, dt: float):
         lib().orc_set_noise_dt(self.h, float(dt))
+
+    def set_inflow(self, inflow_def):
+        """x-low inflow exterior state, numpy (nQ, nface, nz, ny) C order
+        (the reference's Qxlo_ext_def(ny,nz,nface,nQ))."""
+        a = np.ascontiguousarray(inflow_def, dtype=np.float64)
+        self._inflow = a
+        self._check(lib().orc_set_inflow(self.h, _p(a)))
+
+    def set_mask(self, mask, rho_mask: float = 1.25):
+        """Solid cells, numpy (nz, ny, nx) bool; None clears."""
+        if mask is None:
+            self._check(lib().orc_set_mask(self.h, None, 0.0))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self._check(lib().orc_set_mask(self.h, m.ctypes.data_as(C.c_void_p), float(rho_mask)))
+
+    def apply_mask(self, q):
+        self._check(lib().orc_apply_mask(self.h, _p(q)))
 
     def noise_stress(self, stream: int, ids, amp_eta: float, amp_zeta: float) -> np.ndarray:
         """(len(ids), 6) stresses xx,yy,zz,xy,xz,yz at the current step index."""

@@ -14,7 +14,7 @@ from hermeshd_amd import ics
 from hermeshd_amd.config import poiseuille_2d
 
 from tests.test_gpu_parity import field_rel_err, smooth_state
-from tests.test_torch_transcription import CFGS
+from tests.test_torch_transcription import CFGS, inflow_of
 from tests.torch_dg import DG
 
 pytestmark = pytest.mark.gpu
@@ -54,15 +54,18 @@ def test_rhs_vjp_matches_autograd(dev, oracle_mod, name):
     print(f"[rhs_vjp] {name}", flush=True)
     q = _state(cfg, 5)
     lam = np.random.default_rng(6).standard_normal(q.shape)
-    dg = DG(cfg, oracle_mod.Oracle(cfg.as_ints()).tables())
+    dg = DG(cfg, oracle_mod.Oracle(cfg.as_ints()).tables(), inflow=inflow_of(cfg))
     qt = torch.from_numpy(q).requires_grad_(True)
     (ref,) = torch.autograd.grad(dg.rhs(qt), qt, torch.from_numpy(lam))
     s = Solver(cfg)
+    if cfg.bc[0] == "inflow":
+        s.set_inflow(inflow_of(cfg))
     got = s.rhs_vjp(torch.from_numpy(q).to(dev), torch.from_numpy(lam).to(dev)).cpu().numpy()
     assert field_rel_err(got, ref.numpy()) < VJP_TOL
 
 
 @pytest.mark.parametrize("name,stage", [("nb4_nq11_hllc_noslip_ivis1", 2), ("nb4_nq11_hllc_walls_ivis0", 1),
+                                        ("nb4_nq11_hllc_inflow_ivis1", 3),
                                         ("nb10_nq5_hllc_periodic", 3), ("nb8_nq5_central_outflow", 2)])
 def test_stage_vjp_matches_autograd(dev, oracle_mod, name, stage):
     """Transpose of one stage, limiter on with active cells (frozen factors),
@@ -77,7 +80,7 @@ def test_stage_vjp_matches_autograd(dev, oracle_mod, name, stage):
     acc = rng.standard_normal(raw.shape)
     a, b = {1: (0.0, 1.0), 2: (0.75, 0.25), 3: (1.0 / 3.0, 2.0 / 3.0)}[stage]
     dt = 2e-3
-    dg = DG(cfg, oracle_mod.Oracle(cfg.as_ints()).tables())
+    dg = DG(cfg, oracle_mod.Oracle(cfg.as_ints()).tables(), inflow=inflow_of(cfg))
     f, _, th_r, th_p = dg.limit_factors(torch.from_numpy(raw))
     assert bool(f.any()) and bool((th_r < 1).any()) and bool((th_p < 1).any()), "every limiter branch should act"
     rt = torch.from_numpy(raw).requires_grad_(True)
@@ -87,6 +90,8 @@ def test_stage_vjp_matches_autograd(dev, oracle_mod, name, stage):
     g_coll = torch.zeros(()) if g_coll is None else g_coll
     # the same through the kernel: limit a copy, run the stage forward for out, then the transpose
     s = Solver(cfg)
+    if cfg.bc[0] == "inflow":
+        s.set_inflow(inflow_of(cfg))
     lim_d = torch.from_numpy(raw).to(dev)
     qn_d = torch.from_numpy(qn).to(dev)
     s.limit(lim_d)

@@ -361,3 +361,113 @@ def test_ten_moment_has_no_adjoint_or_ensemble(torch_mod):
         s.rhs_vjp(q, torch_mod.ones_like(q))
     with pytest.raises(HDError):
         Ensemble(cfg, [(cfg.nx, cfg.ny, cfg.nz)] * 2)
+
+
+PIPE_CASES = {
+    "pipe_nb4_hllc_nq11": HDConfig(nx=12, ny=8, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, coll=50.0, colvis=0.5,
+                                   riemann="hllc", bc=("inflow", "outflow", "noslip", "noslip", "periodic",
+                                                       "periodic"), lx=1.5, ly=1.0, lz=0.125, T_floor=1e-4),
+    "pipe_nb10_roe_nq5": HDConfig(nx=6, ny=5, nz=3, nbasis=10, iquad=3, nQ=5, riemann="roe",
+                                  bc=("inflow", "outflow", "wall", "wall", "noslip", "noslip"),
+                                  lx=1.2, ly=1.0, lz=0.6, T_floor=1e-4),
+    "pipe_nb8_central_tenmom": HDConfig(nx=6, ny=5, nz=2, nbasis=8, iquad=2, nQ=11, ivis=2, coll=5.0,
+                                        riemann="central", bc=("inflow", "outflow", "noslip", "noslip", "periodic",
+                                                               "periodic"), lx=1.2, ly=1.0, lz=0.4, T_floor=1e-4),
+}
+
+
+@pytest.mark.parametrize("name", list(PIPE_CASES))
+def test_inflow_parity(torch_mod, oracle_mod, name):
+    """x-low inflow with the parabolic profile of set_xlobc_inflow: rhs and
+    4 steps against the oracle."""
+    from hermeshd_amd.solver import Solver
+    cfg = PIPE_CASES[name]
+    print(f"[inflow_parity] {name}", flush=True)
+    inflow = ics.inflow_profile(cfg, 0.3, 1.0, 1.0)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    o.set_inflow(inflow)
+    s = Solver(cfg)
+    with pytest.raises(Exception):
+        s.rhs(_dev(smooth_state(cfg, seed=1)))       # no inflow state yet: refused
+    s.set_inflow(inflow)
+    q = smooth_state(cfg, seed=1)
+    assert field_rel_err(s.rhs(_dev(q)).cpu().numpy(), o.rhs(q)) < TOL
+    qd = _dev(q)
+    q1d = torch_mod.zeros_like(qd)
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    t, dt = np.array(0.0), np.array(0.0)
+    for _ in range(4):
+        o.step(q, q1, q2, t, dt)
+        s.step(qd, q1d)
+    s.check()
+    assert field_rel_err(qd.cpu().numpy(), q) < TOL
+
+
+def test_cylinder_mask_parity(torch_mod, oracle_mod):
+    """icid 7 set-up (cylinder in a pipe, peak inflow 1.5) with the solid mask
+    applied after every step: 5 steps against the oracle; the adjoint refuses
+    a masked context.  The density is jittered by 1e-3: the exactly
+    symmetric set-up has zero normal velocity on the centre plane, where the
+    11-field HLLC flux jumps by the shear stress difference (the stress is
+    not in the star state) and roundoff picks the side."""
+    from hermeshd_amd._lib import HDError
+    from hermeshd_amd.solver import Solver
+    cfg = HDConfig(nx=24, ny=8, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, riemann="hllc",
+                   bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"), lx=3.0, ly=1.0, lz=0.125,
+                   T_floor=0.5)
+    cfg, q, inflow, mask = ics.setup_problem(cfg, 7, vis=0.05)
+    q[0, 0] *= 1.0 + 1e-3 * np.random.default_rng(5).standard_normal(q[0, 0].shape)
+    print(f"[mask_parity] solid cells {int(mask.sum())}", flush=True)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    o.set_inflow(inflow)
+    o.set_mask(mask, 1.25)
+    s = Solver(cfg)
+    s.set_inflow(inflow)
+    s.set_mask(mask, 1.25)
+    qd = _dev(q)
+    q1d = torch_mod.zeros_like(qd)
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    t, dt = np.array(0.0), np.array(0.0)
+    for _ in range(5):
+        o.step(q, q1, q2, t, dt)
+        s.step(qd, q1d)
+    s.check()
+    got = qd.cpu().numpy()
+    assert field_rel_err(got, q) < TOL
+    m = np.broadcast_to(mask, q.shape[2:])
+    assert np.all(got[0, 0][m] == 1.25) and np.all(got[:, 1:4][:, :, m] == 0)
+    with pytest.raises(HDError):
+        s.stage_vjp(qd, torch_mod.ones_like(qd), torch_mod.empty_like(qd), 1e-3, 0.0, 1.0)
+
+
+def test_front_end_reference_problems(torch_mod, oracle_mod):
+    """configure(icid=5..7, vis=...) sets the reference's coll/colvis, the
+    inflow state and (on request) the cylinder mask; the front end's steps
+    match the oracle driven with the same set-up."""
+    from hermeshd_amd import hermeshd as H
+    for icid in (5, 6):
+        # ny odd: no face on the symmetry plane (see test_cylinder_mask_parity)
+        cfg = HDConfig(nx=16, ny=9, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, riemann="hllc",
+                       bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"), lx=2.0, ly=1.0,
+                       lz=0.125, T_floor=0.5)
+        print(f"[front_end_problem] icid {icid}", flush=True)
+        H.configure(cfg, icid=icid, vis=0.05, cylinder=True)
+        q_io = np.zeros((cfg.nx, cfg.ny, cfg.nz, cfg.nQ, cfg.nbasis), order="F")
+        q1 = np.zeros_like(q_io); q2 = np.zeros_like(q_io)
+        t, dt, t1, t_start, dtout = (np.array(0.0) for _ in range(5))
+        nout = np.array(0)
+        H.hermeshd.setup(q_io, t, dt, t1, t_start, dtout, nout)
+        cfg2, q, inflow, mask = ics.setup_problem(cfg, icid, vis=0.05)
+        assert abs(cfg2.coll - 10.0) < 1e-12
+        o = oracle_mod.Oracle(cfg2.as_ints())
+        o.set_inflow(inflow)
+        if mask is not None:
+            o.set_mask(mask, 1.25)
+        np.testing.assert_array_equal(q_io.T, q)
+        qq1, qq2 = np.zeros_like(q), np.zeros_like(q)
+        tt, dd = np.array(0.0), np.array(0.0)
+        for _ in range(3):
+            H.hermeshd.step(q_io, q1, q2, t, dt)
+            o.step(q, qq1, qq2, tt, dd)
+        H.hermeshd.cleanup(t_start)
+        assert field_rel_err(np.ascontiguousarray(q_io.T), q) < TOL

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from hermeshd_amd import ics
-from hermeshd_amd.config import HDConfig, poiseuille_2d, sod_1d, vortex_3d
+from hermeshd_amd.config import EN, MX, MY, MZ, RH, HDConfig, poiseuille_2d, sod_1d, vortex_3d
 from tests import refimpl
 
 BASES = [(4, 2), (8, 2), (10, 3)]
@@ -583,3 +583,121 @@ def test_ten_moment_conserves_totals(oracle_mod):
     r = o.rhs(q)
     tot = r[0].reshape(11, -1).sum(axis=1)
     assert np.abs(tot[:5]).max() < 1e-12 * np.abs(r[0]).max()
+
+
+# ------------------------------------------- inflow boundary, solid mask ----
+def _pipe_cfg(**kw):
+    base = dict(nx=5, ny=4, nz=2, nbasis=4, iquad=2, nQ=11, ivis=1, coll=10.0, colvis=0.5, riemann="hllc",
+                bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"), lx=1.0, ly=0.8, lz=0.4,
+                T_floor=1e-4, limiter=False)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+@pytest.mark.parametrize("riemann", ["hllc", "roe", "central"])
+def test_inflow_ghost_equals_a_periodic_neighbour(oracle_mod, riemann):
+    """x_bc_inflow: the exterior state of the x-low face is (rho, mx, 0, 0, E)
+    from the prescribed state and the stress fields of the interior.  The
+    first cell column then sees exactly what it sees in a periodic box whose
+    last column holds that state as constant cells (with the first column's
+    constant stress): the two right-hand sides agree bitwise there."""
+    cfg = _pipe_cfg(riemann=riemann)
+    rng = np.random.default_rng(3)
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.2, 0.05, 0.0))
+    q[1:, :5] = 0.02 * rng.standard_normal(q[1:, :5].shape)
+    q[0, 5:] = 0.01 * rng.standard_normal(q[0, 5:].shape)       # stress: cell constants only
+    q[1:, 5:] = 0.0
+    inflow = ics.inflow_profile(cfg, 0.3, 1.1, 0.9)
+    oa = oracle_mod.Oracle(cfg.as_ints())
+    oa.set_inflow(inflow)
+    ra = oa.rhs(q)
+    cfgb = cfg.replace(bc=("periodic", "periodic", "noslip", "noslip", "periodic", "periodic"))
+    qb = q.copy()
+    qb[:, :, :, :, -1] = 0.0
+    qb[0, RH, :, :, -1] = inflow[RH, 0].astype(float)
+    qb[0, MX, :, :, -1] = inflow[MX, 0]
+    qb[0, EN, :, :, -1] = inflow[EN, 0]
+    qb[0, 5:, :, :, -1] = q[0, 5:, :, :, 0]
+    rb = oracle_mod.Oracle(cfgb.as_ints()).rhs(qb)
+    np.testing.assert_array_equal(ra[:, :, :, :, 0], rb[:, :, :, :, 0])
+    # and the prescribed state matters
+    inflow2 = inflow.copy()
+    inflow2[RH] *= 1.3
+    oa.set_inflow(inflow2)
+    assert np.abs(oa.rhs(q)[:, :, :, :, 0] - ra[:, :, :, :, 0]).max() > 1e-6
+
+
+def test_inflow_requires_its_state(oracle_mod):
+    o = oracle_mod.Oracle(_pipe_cfg().as_ints())
+    with pytest.raises(oracle_mod.OracleError):
+        o.rhs(ics.uniform(_pipe_cfg()))
+    with pytest.raises(ValueError):
+        _pipe_cfg(bc=("outflow", "inflow", "noslip", "noslip", "periodic", "periodic"))
+
+
+def test_inflow_profile_is_parabolic():
+    cfg = _pipe_cfg(ny=64)
+    d = ics.inflow_profile(cfg, 0.3, 1.0, 2.0)
+    _, yc, _ = ics.cell_centres(cfg)
+    y = yc + cfg.ly / 2
+    u = d[MX, 0, 0] / d[RH, 0, 0]
+    np.testing.assert_allclose(u, 0.3 * 4 * y * (cfg.ly - y) / cfg.ly ** 2, rtol=1e-14)
+    assert abs(u.max() - 0.3) < 1e-3 and u.min() > 0
+    np.testing.assert_allclose(d[EN, 0, 0], 2.0 / (cfg.gamma - 1) + 0.5 * u ** 2, rtol=1e-14)
+    assert np.all(d[MY] == 0) and np.all(d[MZ] == 0)
+    # the same at every face point and z
+    assert np.all(d == d[:, :1, :1])
+
+
+def test_mask_resets_solid_cells_after_each_step(oracle_mod):
+    cfg = _pipe_cfg(nx=8, ny=8, nz=1, bc=("periodic",) * 6, limiter=True)
+    mask = ics.cylinder_mask(cfg, 0.0, 0.0, 0.2)
+    assert 0 < mask.sum() < mask.size
+    rng = np.random.default_rng(1)
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.3, 0.0, 0.0))
+    q[1:, :5] = 0.01 * rng.standard_normal(q[1:, :5].shape)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    ref = q.copy()
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    o.update(ref, q1, q2, 0.01)
+    o.set_mask(mask, 1.25)
+    got = q.copy()
+    o.update(got, q1, q2, 0.01)
+    m = np.broadcast_to(mask, q.shape[2:])
+    np.testing.assert_array_equal(got[:, :, ~m], ref[:, :, ~m])      # fluid cells: the unmasked update
+    assert np.all(got[0, RH][m] == 1.25) and np.all(got[1:, RH][:, m] == 0)
+    assert np.all(got[:, MX:MZ + 1][:, :, m] == 0)
+    np.testing.assert_array_equal(got[:, EN][:, m], ref[:, EN][:, m])  # energy untouched
+    np.testing.assert_array_equal(got[:, 5:][:, :, m], ref[:, 5:][:, :, m])
+
+
+def test_cylinder_mask_area_and_reference_problems():
+    cfg = HDConfig(nx=128, ny=64, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, lx=4.0, ly=1.0, lz=0.1, T_floor=0.5,
+                   bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"))
+    q, inflow, mask = ics.pipe_cylinder_2d(cfg, 1)
+    dx, dy = cfg.lx / cfg.nx, cfg.ly / cfg.ny
+    area = mask.sum() * dx * dy
+    assert abs(area - np.pi * (cfg.ly / 8) ** 2) < 0.05 * np.pi * (cfg.ly / 8) ** 2
+    xc, yc, _ = ics.cell_centres(cfg)
+    jj, ii = np.nonzero(mask[0])
+    cx, cy = xc[ii].mean(), yc[jj].mean()
+    assert abs(cx - (-cfg.lx / 2 + cfg.ly / 2)) < dx and abs(cy) < dy      # centred ly/2 from the walls
+    assert abs((inflow[MX, 0, 0] / inflow[RH, 0, 0]).max() - 1.5) < 1e-2     # version 1: peak 1.5
+    assert np.all(q[0, MX:MZ + 1] == 0) and np.allclose(q[0, EN], 0.5 / (cfg.gamma - 1))
+    c5, q5, in5, m5 = ics.setup_problem(cfg, 5, vis=0.01)
+    assert m5 is None and abs(c5.coll - 0.5 / 0.01) < 1e-12 and abs(c5.colvis - 0.5) < 1e-15
+    assert abs((in5[MX, 0, 0] / in5[RH, 0, 0]).max() - 0.3) < 1e-2
+    c2, _, _, _ = ics.setup_problem(cfg.replace(ivis=0, nQ=5, coll=3.0), 2, vis=0.01)
+    assert c2.coll == 0.0 and c2.colvis == 0.0
+
+
+def test_hydro_jet_closed_form():
+    cfg = HDConfig(nx=8, ny=32, nz=1, nbasis=4, iquad=2, nQ=5, lx=2.0, ly=2.0, lz=0.1, T_floor=0.1)
+    q = ics.set_ic(cfg, 0)
+    _, yc, _ = ics.cell_centres(cfg)
+    ch = np.cosh(20 * yc / (cfg.ly / 2))
+    np.testing.assert_allclose(q[0, MX, 0, :, 3], 1.0 / ch, rtol=1e-14)
+    assert np.all(np.abs(q[0, MY] * ch[None, :, None] ** 2) <= 0.0025 + 1e-15)
+    e = 0.1 / (cfg.gamma - 1) + 0.5 * (q[0, MX] ** 2 + q[0, MY] ** 2)
+    np.testing.assert_allclose(q[0, EN], e, rtol=1e-14)
+    assert np.all(q[1:] == 0)

@@ -23,17 +23,35 @@ CFGS = {
     "nb10_nq11_hllc_mixed": HDConfig(nx=3, ny=4, nz=3, nbasis=10, iquad=3, nQ=11, ivis=1, coll=30.0, colvis=0.5,
                                      riemann="hllc", bc=("outflow", "outflow", "wall", "wall", "noslip", "noslip"),
                                      lx=0.9, ly=1.2, lz=0.9, T_floor=1e-4),
+    # the pipe of icid 5: parabolic inflow, outflow, no-slip walls
+    "nb4_nq11_hllc_inflow_ivis1": HDConfig(nx=5, ny=6, nz=1, nbasis=4, iquad=2, nQ=11, ivis=1, coll=10.0,
+                                           colvis=0.5, riemann="hllc",
+                                           bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"),
+                                           lx=1.0, ly=1.2, lz=0.2, T_floor=1e-4),
 }
+
+
+def inflow_of(cfg):
+    """The x-low inflow state used with an inflow configuration (peak 0.3,
+    rho = 1, p = 1), or None."""
+    return ics.inflow_profile(cfg, 0.3, 1.0, 1.0) if cfg.bc[0] == "inflow" else None
+
+
+def oracle_for(oracle_mod, cfg):
+    o = oracle_mod.Oracle(cfg.as_ints())
+    if cfg.bc[0] == "inflow":
+        o.set_inflow(inflow_of(cfg))
+    return o
 
 
 @pytest.mark.parametrize("name", list(CFGS))
 def test_transcription_rhs_matches_oracle(oracle_mod, name):
     cfg = CFGS[name]
-    o = oracle_mod.Oracle(cfg.as_ints())
+    o = oracle_for(oracle_mod, cfg)
     q = smooth_state(cfg, seed=3)
     if "outflow" in name:
         q[0, 1] += 0.2 * np.sign(np.random.default_rng(1).standard_normal(q[0, 1].shape))
-    dg = DG(cfg, o.tables())
+    dg = DG(cfg, o.tables(), inflow=inflow_of(cfg))
     r = dg.rhs(torch.from_numpy(q)).numpy()
     assert field_rel_err(r, o.rhs(q)) < 1e-12
 

@@ -22,7 +22,7 @@ import torch
 
 RH, MX, MY, MZ, EN = 0, 1, 2, 3, 4
 PXX, PYY, PZZ, PXY, PXZ, PYZ = 5, 6, 7, 8, 9, 10
-BC = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3}
+BC = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3, "inflow": 4}
 
 
 def degrees(nb):
@@ -44,8 +44,10 @@ def _dL(a, x):
 
 
 class DG:
-    def __init__(self, cfg, tables, dtype=torch.float64, device="cpu"):
+    def __init__(self, cfg, tables, dtype=torch.float64, device="cpu", inflow=None):
+        """inflow: x-low inflow state (nQ, nface, nz, ny) when cfg.bc[0] is "inflow"."""
         self.cfg = cfg
+        self.inflow = None if inflow is None else torch.as_tensor(np.asarray(inflow), dtype=dtype, device=device)
         self.nb, self.nQ = cfg.nbasis, cfg.nQ
         q = cfg.iquad
         t = tables
@@ -165,7 +167,15 @@ class DG:
     def _ghost(self, own, d, side):
         kind = self.bc[2 * d + side]
         ex = own.clone()
-        if kind == BC["noslip"]:
+        if kind == BC["inflow"]:
+            # x_bc_inflow (src/boundary.f90:571-592): own is (nface, nQ, nz, ny, 1)
+            g = self.inflow.permute(1, 0, 2, 3).unsqueeze(-1)
+            ex[:, RH] = g[:, RH]
+            ex[:, MX] = g[:, MX]
+            ex[:, MY] = 0.0
+            ex[:, MZ] = 0.0
+            ex[:, EN] = g[:, EN]
+        elif kind == BC["noslip"]:
             ex[:, MX:MZ + 1] = 0.0
         elif kind == BC["wall"]:
             ex[:, MX + d] = 0.0
