@@ -38,6 +38,7 @@ class HDConfigC(C.Structure):
         ("rh_floor", C.c_double), ("T_floor", C.c_double), ("coll", C.c_double), ("colvis", C.c_double),
         ("cflm", C.c_double), ("body_force", C.c_double * 3), ("eta_sd", C.c_double), ("zeta_sd", C.c_double),
         ("seed", C.c_uint64), ("realization", C.c_uint32), ("reserved1", C.c_uint32),
+        ("eos_p1", C.c_double), ("eos_pbase", C.c_double),
     ]
 
 

@@ -31,6 +31,14 @@ def reference_T_floor(TK: float = 300.0, mu: float = 18.0) -> float:
     return (TK * eV_per_K) / te0
 
 
+def reference_tait(mu: float = 39.948):
+    """(P_1, P_base) of the Murnaghan-Tait water EOS in code units
+    (src/params.f90:97-105): 2.15e9/7.2/p0 and 1.01e5/p0 with
+    p0 = mu*1.67e-27*n0*v0^2 (mu of the shipped src/input.f90:69)."""
+    p0 = mu * 1.67e-27 * 2.5e25 * 1.0 ** 2
+    return 2.15e9 / 7.2 / p0, 1.01e5 / p0
+
+
 def default_cflm(iquad: int) -> float:
     """set_cflm (src/initialize.f90:104-122)."""
     return {2: 0.14, 3: 0.08, 4: 0.05}[iquad]
@@ -65,6 +73,8 @@ class HDConfig:
     zeta_sd: float = 0.0
     seed: int = 1234
     realization: int = 0
+    eos_p1: float = field(default_factory=lambda: reference_tait()[0])
+    eos_pbase: float = field(default_factory=lambda: reference_tait()[1])
 
     def __post_init__(self):
         if (self.nbasis, self.iquad) not in VALID_BASIS:
@@ -80,8 +90,10 @@ class HDConfig:
             raise ValueError("ivis=2 has no stochastic stress")
         if self.noise and self.nQ != 11:
             raise ValueError("stochastic stress needs nQ=11")
-        if self.ieos != 1:
-            raise ValueError("only ieos=1 (ideal gas) is supported")
+        if self.ieos not in (1, 2):
+            raise ValueError("ieos must be 1 (ideal gas) or 2 (Murnaghan-Tait water)")
+        if self.ieos == 2 and self.riemann == "roe":
+            raise ValueError("ieos=2 runs with the HLLC or central flux (the Roe solver is ideal-gas only)")
         if self.riemann not in RIEMANN_NAMES:
             raise ValueError(f"riemann must be one of {list(RIEMANN_NAMES)}")
         if self.integrator not in INTEGRATOR_NAMES:
@@ -132,6 +144,7 @@ class HDConfig:
             T_floor=self.T_floor, coll=self.coll, colvis=self.colvis, cflm=self.cflm,
             body_force=list(self.body_force), eta_sd=self.eta_sd, zeta_sd=self.zeta_sd,
             seed=int(self.seed), realization=int(self.realization),
+            eos_p1=float(self.eos_p1), eos_pbase=float(self.eos_pbase),
         )
 
 

@@ -45,7 +45,9 @@ int validate(const hd_config* c) {
     if (c->ivis < 0 || c->ivis > 2) return fail(HD_ERR_INVALID, "ivis must be 0, 1 or 2");
     if (c->ivis >= 1 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=1,2 need nQ=11");
     if (c->ivis == 2 && c->noise) return fail(HD_ERR_INVALID, "ivis=2 has no stochastic stress");
-    if (c->ieos != 1) return fail(HD_ERR_INVALID, "only ieos=1 (ideal gas) is supported");
+    if (c->ieos != 1 && c->ieos != 2) return fail(HD_ERR_INVALID, "ieos must be 1 (ideal gas) or 2 (water)");
+    if (c->ieos == 2 && c->riemann == HD_RIEMANN_ROE)
+        return fail(HD_ERR_INVALID, "ieos=2 runs with the HLLC or central flux");
     if (c->riemann < 0 || c->riemann > 2) return fail(HD_ERR_INVALID, "riemann must be 0, 1 or 2");
     if (c->integrator != 2 && c->integrator != 3) return fail(HD_ERR_INVALID, "integrator must be 2 or 3");
     for (int d = 0; d < 3; d++) {
@@ -153,6 +155,15 @@ int apply_mask(hd_ctx* c, double* q, hipStream_t s) {
     return HD_OK;
 }
 
+CflEos cfl_eos(const hd_ctx* c) {
+    CflEos e;
+    e.gamma = c->cfg.gamma;
+    e.ieos = c->cfg.ieos;
+    e.P1 = c->cfg.eos_p1;
+    e.T_floor = c->cfg.T_floor;
+    return e;
+}
+
 int advance_step(hd_ctx* c, hipStream_t s) {
     if (!c->cfg.noise) return HD_OK;
     hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, c->step_ctr, 0ull, 1);
@@ -232,6 +243,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     ph.P_floor = cfg->T_floor * cfg->rh_floor; ph.rh_floor = cfg->rh_floor;
     ph.colvis = cfg->colvis; ph.c2d3cv = (2. / 3.) * cfg->colvis; ph.c4d3cv = (4. / 3.) * cfg->colvis;
     ph.ivis = cfg->ivis;
+    ph.ieos = cfg->ieos; ph.P1 = cfg->eos_p1; ph.Pbase = cfg->eos_pbase;
     c->lp.rh_floor = cfg->rh_floor; c->lp.epsP = cfg->rh_floor * cfg->T_floor; c->lp.gm1 = ph.gm1;
     build_tables(c->nb, c->q, &c->htab);
     int r = HD_OK;
@@ -282,8 +294,9 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
     const char* kern = std::getenv("HD_KERNELS");
-    // the full 10-moment equations (ivis 2) run on the table-driven kernels only
-    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2;
+    // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
+    // table-driven kernels only
+    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2;
     if (c->generic && cfg->noise) {
         hd_destroy(c);
         return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
@@ -372,9 +385,9 @@ int hd_min_dt(hd_ctx* c, const double* q, double* dt_dev, void* stream) {
     if (!c || !q || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
     const int nblk = (int)std::min<long>(c->npart, (c->N + 255) / 256);
-    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q, c->grid, c->cfg.gamma, 1.01 * c->cfg.rh_floor,
+    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q, c->grid, cfl_eos(c), 1.01 * c->cfg.rh_floor,
                        c->partial);
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q, c->grid, c->cfg.gamma, c->partial, nblk, c->cfg.cflm,
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q, c->grid, cfl_eos(c), c->partial, nblk, c->cfg.cflm,
                        dt_dev, (double*)nullptr);
     HIPCHK(hipGetLastError());
     return HD_OK;
@@ -531,9 +544,9 @@ int hd_step(hd_ctx* c, double* q_io, double* q1, double* q2, double* t_dev, doub
     if (!c || !q_io || !q1 || !dt_dev) return fail(HD_ERR_INVALID, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
     const int nblk = (int)std::min<long>(c->npart, (c->N + 255) / 256);
-    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q_io, c->grid, c->cfg.gamma, 1.01 * c->cfg.rh_floor,
+    hipLaunchKernelGGL(k_wavespeed, dim3(nblk), dim3(256), 0, s, q_io, c->grid, cfl_eos(c), 1.01 * c->cfg.rh_floor,
                        c->partial);
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q_io, c->grid, c->cfg.gamma, c->partial, nblk,
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, q_io, c->grid, cfl_eos(c), c->partial, nblk,
                        c->cfg.cflm, dt_dev, t_dev);
     HIPCHK(hipGetLastError());
     return hd_update(c, q_io, q1, q2, dt_dev, stream);

@@ -182,7 +182,8 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     if (nreal < 1) return fail(HD_ERR_INVALID, "nreal must be >= 1");
     int rc = validate(base);
     if (rc) return rc;
-    if (base->ivis == 2) return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (ivis 0 or 1)");
+    if (base->ivis == 2 || base->ieos == 2)
+        return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (ivis 0 or 1, ieos 1)");
     if (base->bc[0] == HD_BC_INFLOW) return fail(HD_ERR_UNSUPPORTED, "ensembles take no inflow boundary");
     for (int r = 0; r < nreal; r++) {
         hd_config c = *base;
@@ -209,6 +210,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     ph.P_floor = base->T_floor * base->rh_floor; ph.rh_floor = base->rh_floor;
     ph.colvis = base->colvis; ph.c2d3cv = (2. / 3.) * base->colvis; ph.c4d3cv = (4. / 3.) * base->colvis;
     ph.ivis = base->ivis;
+    ph.ieos = base->ieos; ph.P1 = base->eos_p1; ph.Pbase = base->eos_pbase;
     e->lp.rh_floor = base->rh_floor; e->lp.epsP = base->rh_floor * base->T_floor; e->lp.gm1 = ph.gm1;
     Tables t;
     build_tables(e->nb, e->q, &t);

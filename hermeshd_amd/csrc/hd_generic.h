@@ -173,7 +173,10 @@ __device__ __forceinline__ void gflux(const Phys& ph, const double* Q, double* F
             return;
         }
     }
-    flux_point<NQ, DIR>(ph, Q, F, nullptr);
+    if (ph.ieos == 2)
+        flux_point<NQ, DIR, true>(ph, Q, F, nullptr);
+    else
+        flux_point<NQ, DIR>(ph, Q, F, nullptr);
 }
 
 template <int NB, int NF, int NQ>
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q,
                 for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]) - 0.5 * cw * (QR[p][f] - QL[p][f]);
             }
         }
-        if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL[p], QR[p], FL, FR, out);
+        if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL[p], QR[p], FL, FR, out, 1.0 / QR[NF - 1][RH]);
         if constexpr (RM == RIEMANN_ROE) roe_point<DIR, NQ>(ph, QL[p], QR[p], FL, FR, out);
 #pragma unroll
         for (int f = 0; f < NQ; f++) fl[((long)f * NF + p) * nfaces + fid] = out[f];
@@ -525,12 +528,20 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 // cells with rho > rh_mult*rh_floor, then a single-block finish that adds the
 // sound speed of the last cell, as the reference does.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double cell_sound_speed(const double* q, const Grid& g, long c, double gamma,
+// sound speed of get_min_dt: ideal gas, or the water EOS (src/hermeshd.f90:165-166)
+struct CflEos {
+    double gamma;
+    int ieos;
+    double P1, T_floor;
+};
+
+__device__ __forceinline__ double cell_sound_speed(const double* q, const Grid& g, long c, const CflEos& e,
                                                    double* vmag0) {
     const double dn = q[qoff(g, c, RH, 0)];
     const double dni = 1. / dn;
     const double vx = q[qoff(g, c, MX, 0)] * dni, vy = q[qoff(g, c, MY, 0)] * dni, vz = q[qoff(g, c, MZ, 0)] * dni;
-    const double cs = sqrt(gamma * (q[qoff(g, c, EN, 0)] * dni - 0.5 * (vx * vx + vy * vy + vz * vz)));
+    const double cs = e.ieos == 2 ? sqrt(7.2 * e.P1 * pow(dn, 6.2) + e.T_floor)
+                                  : sqrt(e.gamma * (q[qoff(g, c, EN, 0)] * dni - 0.5 * (vx * vx + vy * vy + vz * vz)));
     *vmag0 = fmax(fmax(fabs(vx) + cs, fabs(vy) + cs), fabs(vz) + cs);
     return cs;
 }
@@ -541,13 +552,13 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-static __global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q, Grid g, double gamma,
+static __global__ __launch_bounds__(256) void k_wavespeed(const double* __restrict__ q, Grid g, CflEos eos,
                                                    double dn_min, double* __restrict__ partial) {
     __shared__ double red[4];
     double vmag = 0.;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
         double v0;
-        cell_sound_speed(q, g, c, gamma, &v0);
+        cell_sound_speed(q, g, c, eos, &v0);
         const double dn = q[qoff(g, c, RH, 0)];
         if (v0 > vmag && dn > dn_min) vmag = v0;
     }
@@ -561,7 +572,7 @@ static __global__ __launch_bounds__(256) void k_wavespeed(const double* __restri
     }
 }
 
-static __global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ q, Grid g, double gamma,
+static __global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ q, Grid g, CflEos eos,
                                                    const double* __restrict__ partial, int npart, double cflm,
                                                    double* __restrict__ dt_out, double* __restrict__ t_io) {
     __shared__ double red[4];
@@ -574,7 +585,7 @@ static __global__ __launch_bounds__(256) void k_dt_finish(const double* __restri
         double vmag = red[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) vmag = fmax(vmag, red[w]);
         double v0;
-        const double cs_last = cell_sound_speed(q, g, g.N - 1, gamma, &v0);
+        const double cs_last = cell_sound_speed(q, g, g.N - 1, eos, &v0);
         const double vmax = (vmag + cs_last) * g.dxi;
         const double dt = cflm / vmax;
         *dt_out = dt;

@@ -4,8 +4,8 @@
 // per-face loops of the reference are independent across points, so a
 // per-point function computes the same values.
 //
-//   flux_point  : flux_calc_pnts_r  (src/flux.f90:23-193), ivis in {0,1}, ieos=1
-//   hllc_point  : flux_hllc         (src/flux.f90:466-647), ibatten=0 (Davis speeds)
+//   flux_point  : flux_calc_pnts_r  (src/flux.f90:23-193), ivis in {0,1}, ieos 1 (2 with TAIT)
+//   hllc_point  : flux_hllc         (src/flux.f90:466-647), ibatten=0 (Davis speeds), ieos 1/2
 //   roe_point   : flux_roe          (src/ces/10_moment/dg_3D_hydro_lqc_SEAN.f90:1614-1789)
 //   cfcal       : cfcal             (src/flux.f90:1029-1072)
 //   flux_point10, cfcal10 : the ivis = 2 branches of flux_calc_pnts_r and
@@ -26,10 +26,19 @@ struct Phys {
     double P_floor, rh_floor;
     double colvis, c2d3cv, c4d3cv;
     int ivis;                // 2: full 10-moment equations (table-driven kernels only)
+    int ieos;                // 2: Murnaghan-Tait water (table-driven kernels only)
+    double P1, Pbase;        // P = P1*(rho^7.2 - 1) + Pbase + P_ideal when ieos = 2
 };
 
+// Murnaghan-Tait pressure on top of the ideal-gas one (src/flux.f90:64,526,
+// src/params.f90:97-105)
+HD_DEV double tait_pressure(const Phys& ph, double dn, double P) {
+    return ph.P1 * (pow(dn, 7.2) - 1.) + ph.Pbase + P;
+}
+
 // DIR: 0=x, 1=y, 2=z.  S: stochastic stress xx,yy,zz,xy,xz,yz or nullptr.
-template <int NQ, int DIR>
+// TAIT: Murnaghan-Tait water pressure (ieos 2; table-driven kernels only).
+template <int NQ, int DIR, bool TAIT = false>
 HD_DEV void flux_point(const Phys& ph, const double* Q, double* F, const double* S) {
     const double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
     double P_xx = 0., P_yy = 0., P_zz = 0., P_xy = 0., P_xz = 0., P_yz = 0.;
@@ -42,6 +51,7 @@ HD_DEV void flux_point(const Phys& ph, const double* Q, double* F, const double*
     const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
     const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
     double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
+    if constexpr (TAIT) P = tait_pressure(ph, dn, P);
     if (P < ph.P_floor) P = ph.P_floor;
     if constexpr (DIR == 0) {
         F[RH] = M_x;
@@ -89,6 +99,7 @@ HD_DEV void flux_point10(const Phys& ph, const double* Q, double* F) {
     const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
     const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
     double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
+    if (ph.ieos == 2) P = tait_pressure(ph, dn, P);
     if (P < ph.P_floor) P = ph.P_floor;
     double P_10 = (1. / 3.) * (P_xx + P_yy + P_zz - dni * smsq);
     if (P_10 < ph.P_floor) P_10 = ph.P_floor;
@@ -133,7 +144,7 @@ HD_DEV double cfcal10(const Phys& ph, const double* Q) {
     const double dn = Q[RH], dni = 1. / dn;
     const double vx = Q[MX] * dni, vy = Q[MY] * dni, vz = Q[MZ] * dni;
     const double P = (1. / 3.) * (Q[PXX] + Q[PYY] + Q[PZZ] - dn * (vx * vx + vy * vy + vz * vz));
-    const double cs = sqrt(ph.gamma * P * dni);
+    const double cs = ph.ieos == 2 ? sqrt(7.2 * ph.P1 * pow(dn, 6.2)) : sqrt(ph.gamma * P * dni);
     const double v = DIR == 0 ? vx : DIR == 1 ? vy : vz;
     return fabs(v) + cs;
 }
@@ -151,7 +162,7 @@ HD_DEV void dir_fields(int& ipar, int& ip1, int& ip2) {
 // kept when no branch of Eq. (26) applies (only possible with NaN speeds).
 template <int DIR>
 HD_DEV void hllc_point(const Phys& ph, const double* QL, const double* QR, const double* FL,
-                       const double* FR, double* out) {
+                       const double* FR, double* out, double rho_i_last = 0.) {
     int ipar, ip1, ip2;
     dir_fields<DIR>(ipar, ip1, ip2);
     const double gm1 = ph.gm1, g = ph.gamma;
@@ -160,10 +171,21 @@ HD_DEV void hllc_point(const Phys& ph, const double* QL, const double* QR, const
     const double riL = 1.0 / rL, riR = 1.0 / rR;
     const double uL = QL[ipar] * riL, vL = QL[ip1] * riL, wL = QL[ip2] * riL;
     const double uR = QR[ipar] * riR, vR = QR[ip1] * riR, wR = QR[ip2] * riR;
-    const double pL = gm1 * (QL[EN] - 0.5 * rL * (uL * uL + vL * vL + wL * wL));
-    const double pR = gm1 * (QR[EN] - 0.5 * rR * (uR * uR + vR * vR + wR * wR));
+    double pL = gm1 * (QL[EN] - 0.5 * rL * (uL * uL + vL * vL + wL * wL));
+    double pR = gm1 * (QR[EN] - 0.5 * rR * (uR * uR + vR * vR + wR * wR));
     // Davis wave speeds (src/flux.f90:522-585)
-    const double cL = sqrt(g * pL / rL), cR = sqrt(g * pR / rR);
+    double cL, cR;
+    if (ph.ieos == 2) {
+        // water: the reference's sound speed takes rho_i as its loop leaves it,
+        // 1/rho of the face's last right state (rho_i_last) at every point
+        pL = tait_pressure(ph, rL, pL);
+        pR = tait_pressure(ph, rR, pR);
+        cL = sqrt(7.2 * ph.P1 * pow(rL, 6.2) + pL * rho_i_last);
+        cR = sqrt(7.2 * ph.P1 * pow(rR, 6.2) + pR * rho_i_last);
+    } else {
+        cL = sqrt(g * pL / rL);
+        cR = sqrt(g * pR / rR);
+    }
     const double SL = fmin(uL - cL, uR - cR);
     const double SR = fmax(uR + cR, uL + cL);
     // contact speed and star pressure, Eqs. (34),(36) of Batten 1997

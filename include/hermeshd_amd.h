@@ -47,7 +47,8 @@ typedef struct hd_config {
     int32_t nbasis, iquad, nQ;   /* (4,2), (8,2) or (10,3); nQ 5 or 11 */
     int32_t ivis, ieos;          /* ivis 0 explicit, 1 semi-implicit stress relaxation, 2 full
                                     10-moment equations (nQ 11, no noise, no adjoint or ensemble:
-                                    table-driven kernels); ieos 1 */
+                                    table-driven kernels); ieos 1 ideal gas, 2 Murnaghan-Tait
+                                    water (table-driven kernels, no Roe) */
     int32_t riemann;             /* HD_RIEMANN_* (input.f90 ihllc; Roe from the legacy solver) */
     int32_t limiter;             /* 1 = positivity limiter on every stage input */
     int32_t integrator;          /* 2 = heun (SSP-RK2), 3 = shu-osher (SSP-RK3) */
@@ -63,6 +64,8 @@ typedef struct hd_config {
     double eta_sd, zeta_sd;      /* noise amplitudes sqrt(2 eta T), sqrt(zeta T / 3) */
     uint64_t seed;               /* Philox key */
     uint32_t realization, reserved1;
+    double eos_p1, eos_pbase;    /* ieos 2: P = eos_p1*(rho^7.2 - 1) + eos_pbase + P_ideal
+                                    (src/params.f90:97-105: P_1, P_base) */
 } hd_config;
 
 typedef struct hd_ctx hd_ctx;

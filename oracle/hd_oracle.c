@@ -178,6 +178,8 @@ orc_ctx *orc_create(const orc_config *cfg) {
     if (cfg->ivis >= 1 && cfg->nQ != 11) { set_err(-1, "ivis=1,2 need nQ=11"); return NULL; }
     if (cfg->ivis == 2 && cfg->noise) { set_err(-1, "ivis=2 has no stochastic stress"); return NULL; }
     if (cfg->noise && cfg->nQ != 11) { set_err(-1, "noise needs nQ=11"); return NULL; }
+    if (cfg->ieos != 1 && cfg->ieos != 2) { set_err(-1, "ieos must be 1 or 2"); return NULL; }
+    if (cfg->ieos == 2 && cfg->riemann == ORC_FLUX_ROE) { set_err(-1, "ieos=2 has no Roe solver"); return NULL; }
     for (int s = 0; s < 6; s++) {
         if (cfg->bc[s] < 0 || cfg->bc[s] > ORC_BC_INFLOW) { set_err(-1, "unknown boundary kind"); return NULL; }
         if (cfg->bc[s] == ORC_BC_INFLOW && s != 0) { set_err(-1, "inflow is an x-low boundary"); return NULL; }
@@ -353,6 +355,8 @@ static void flux_points_s(const orc_ctx *c, const double *qp, double *fp, int di
         double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
         double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
         double P = c->aindm1 * (Ener - 0.5 * dni * smsq);
+        /* Murnaghan-Tait water (src/flux.f90:64, src/params.f90:97-105) */
+        if (c->c.ieos == 2) P = c->c.eos_p1 * (pow(dn, 7.2) - 1.) + c->c.eos_pbase + P;
         if (P < c->P_floor) P = c->P_floor;
         if (c->c.ivis == 2) {
             /* full 10-moment model (src/flux.f90:64-66,89-117,134-155,172-191):
@@ -449,7 +453,7 @@ double orc_cfcal(const orc_ctx *c, const double *Q, int dir) {
     double vx = Q[MX] * dni, vy = Q[MY] * dni, vz = Q[MZ] * dni;
     double P = c->aindm1 * (Q[EN] - 0.5 * dn * (vx * vx + vy * vy + vz * vz));
     if (c->c.ivis == 2) P = (1. / 3.) * (Q[PXX] + Q[PYY] + Q[PZZ] - dn * (vx * vx + vy * vy + vz * vz));
-    double cs = sqrt(c->aindex * P * dni);
+    double cs = c->c.ieos == 2 ? sqrt(7.2 * c->c.eos_p1 * pow(dn, 6.2)) : sqrt(c->aindex * P * dni);
     double v = (dir == 1) ? vx : (dir == 2) ? vy : vz;
     return fabs(v) + cs;
 }
@@ -461,7 +465,7 @@ static void dir_fields(int dir, int *ipar, int *ip1, int *ip2) {
     else { *ipar = MZ; *ip1 = MX; *ip2 = MY; }
 }
 
-/* flux_hllc (src/flux.f90:466-647) with ibatten = 0, ieos = 1.
+/* flux_hllc (src/flux.f90:466-647) with ibatten = 0, ieos 1 or 2.
    qlr, flr: (nfe x nQ) rows 0..nface-1 left, nface..nfe-1 right; fout: (nface x 5) */
 static void hllc_face(const orc_ctx *c, const double *qlr, const double *flr, double *fout, int dir) {
     const int nf = c->nface, nfe = c->nfe, nQ = c->nQ;
@@ -480,13 +484,27 @@ static void hllc_face(const orc_ctx *c, const double *qlr, const double *flr, do
         v1[k] = a;
         double qsq = a * a + b * b + d * d;
         plr[k] = aindm1 * (Q[EN] - 0.5 * rhov[k] * qsq);
+        if (c->c.ieos == 2) plr[k] = c->c.eos_p1 * (pow(rhov[k], 7.2) - 1.) + c->c.eos_pbase + plr[k];
     }
-    for (int k = 0; k < nf; k++) {
-        int k2 = k + nf;
-        cslr[k] = v1[k] - sqrt(aindex * plr[k] / rhov[k]);
-        cslr[k2] = v1[k2] + sqrt(aindex * plr[k2] / rhov[k2]);
-        qslr[k] = v1[k2] - sqrt(aindex * plr[k2] / rhov[k2]);
-        qslr[k2] = v1[k] + sqrt(aindex * plr[k] / rhov[k]);
+    if (c->c.ieos == 2) {
+        /* the water-EOS wave speeds use rho_i as the loop above leaves it:
+           1/rho of the last right state, for every point (src/flux.f90:519-534,569-571) */
+        const double rho_i = 1.0 / rhov[nfe - 1], p1 = c->c.eos_p1;
+        for (int k = 0; k < nf; k++) {
+            int k2 = k + nf;
+            cslr[k] = v1[k] - sqrt(7.2 * p1 * pow(rhov[k], 6.2) + plr[k] * rho_i);
+            cslr[k2] = v1[k2] + sqrt(7.2 * p1 * pow(rhov[k2], 6.2) + plr[k2] * rho_i);
+            qslr[k] = v1[k2] - sqrt(7.2 * p1 * pow(rhov[k2], 6.2) + plr[k2] * rho_i);
+            qslr[k2] = v1[k] + sqrt(7.2 * p1 * pow(rhov[k], 6.2) + plr[k] * rho_i);
+        }
+    } else {
+        for (int k = 0; k < nf; k++) {
+            int k2 = k + nf;
+            cslr[k] = v1[k] - sqrt(aindex * plr[k] / rhov[k]);
+            cslr[k2] = v1[k2] + sqrt(aindex * plr[k2] / rhov[k2]);
+            qslr[k] = v1[k2] - sqrt(aindex * plr[k2] / rhov[k2]);
+            qslr[k2] = v1[k] + sqrt(aindex * plr[k] / rhov[k]);
+        }
     }
     for (int k = 0; k < nf; k++) {
         int k2 = k + nf;
@@ -1154,7 +1172,10 @@ double orc_min_dt(orc_ctx *c, const double *q) {
                 double vx = q[qidx(c, i, j, k, MX, 0)] * dni;
                 double vy = q[qidx(c, i, j, k, MY, 0)] * dni;
                 double vz = q[qidx(c, i, j, k, MZ, 0)] * dni;
-                cs = sqrt(c->aindex * (q[qidx(c, i, j, k, EN, 0)] * dni - 0.5 * (vx * vx + vy * vy + vz * vz)));
+                if (g->ieos == 2)
+                    cs = sqrt(7.2 * g->eos_p1 * pow(dn, 6.2) + g->T_floor);
+                else
+                    cs = sqrt(c->aindex * (q[qidx(c, i, j, k, EN, 0)] * dni - 0.5 * (vx * vx + vy * vy + vz * vz)));
                 double vmag0 = fmax(fmax(fabs(vx) + cs, fabs(vy) + cs), fabs(vz) + cs);
                 if (vmag0 > vmag && dn > 1.01 * g->rh_floor) vmag = vmag0;
             }

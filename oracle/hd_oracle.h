@@ -22,7 +22,7 @@ enum { ORC_FLUX_CENTRAL = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_ROE = 2 };
 typedef struct orc_config {
     int nx, ny, nz;
     int nbasis, iquad, nQ;       /* (nbasis,iquad) in {(4,2),(8,2),(10,3)}; nQ in {5,11} */
-    int ivis, ieos;              /* ivis in {0,1,2}; ieos = 1 (ideal gas) */
+    int ivis, ieos;              /* ivis in {0,1,2}; ieos 1 (ideal gas) or 2 (Murnaghan-Tait water) */
     int riemann;                 /* ORC_FLUX_* */
     int limiter;                 /* 1: positivity limiter on every stage input */
     int integrator;              /* 2 = heun (SSP-RK2), 3 = shu-osher (SSP-RK3) */
@@ -38,6 +38,7 @@ typedef struct orc_config {
     double eta_sd, zeta_sd;      /* noise amplitudes (stochastic stress) */
     uint64_t seed;               /* noise key */
     uint32_t realization, pad2_;
+    double eos_p1, eos_pbase;    /* ieos = 2: P = P_1*(rho^7.2 - 1) + P_base + P_ideal */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;

@@ -29,6 +29,7 @@ class OrcConfig(C.Structure):
         ("coll", C.c_double), ("colvis", C.c_double), ("cflm", C.c_double),
         ("body_force", C.c_double * 3), ("eta_sd", C.c_double), ("zeta_sd", C.c_double),
         ("seed", C.c_uint64), ("realization", C.c_uint32), ("pad2_", C.c_uint32),
+        ("eos_p1", C.c_double), ("eos_pbase", C.c_double),
     ]
 
 

@@ -75,6 +75,13 @@ CASES = {
     "tenmom_nb8_central_outflow": HDConfig(nx=5, ny=6, nz=4, nbasis=8, iquad=2, nQ=11, ivis=2, coll=2.0,
                                            riemann="central", bc=("outflow",) * 6, lx=1.0, ly=1.2, lz=0.8,
                                            T_floor=1e-4, integrator="shu-osher"),
+    # Murnaghan-Tait water EOS (ieos 2), scaled constants: table-driven kernels
+    "water_nb4_hllc_periodic": HDConfig(nx=6, ny=5, nz=4, nbasis=4, iquad=2, nQ=5, ieos=2, riemann="hllc",
+                                        lx=1.2, ly=1.0, lz=0.8, T_floor=1e-4, eos_p1=2.0, eos_pbase=1.0),
+    "water_nb10_central_walls": HDConfig(nx=5, ny=4, nz=4, nbasis=10, iquad=3, nQ=11, ivis=1, coll=10.0,
+                                         colvis=0.3, ieos=2, riemann="central",
+                                         bc=("wall", "wall", "noslip", "noslip", "periodic", "periodic"),
+                                         lx=1.0, ly=0.8, lz=0.8, T_floor=1e-4, eos_p1=2.0, eos_pbase=1.0),
 }
 
 
@@ -136,7 +143,8 @@ def test_min_dt_parity(torch_mod, oracle_mod, name):
                                          ("nb10_central_outflow", 3), ("relax_nq11_bodyforce", 20),
                                          ("nq11_hllc_xoutflow", 4),
                                          ("relax_nb10_roe", 5), ("tenmom_nb4_hllc_periodic", 6),
-                                         ("tenmom_nb10_roe_walls", 3), ("tenmom_nb8_central_outflow", 4)])
+                                         ("tenmom_nb10_roe_walls", 3), ("tenmom_nb8_central_outflow", 4),
+                                         ("water_nb4_hllc_periodic", 5), ("water_nb10_central_walls", 3)])
 def test_step_parity(torch_mod, oracle_mod, name, nsteps):
     """Reference step semantics (dt from get_min_dt, RK update, t += dt) for N steps."""
     from hermeshd_amd.solver import Solver

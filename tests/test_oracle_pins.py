@@ -701,3 +701,87 @@ def test_hydro_jet_closed_form():
     e = 0.1 / (cfg.gamma - 1) + 0.5 * (q[0, MX] ** 2 + q[0, MY] ** 2)
     np.testing.assert_allclose(q[0, EN], e, rtol=1e-14)
     assert np.all(q[1:] == 0)
+
+
+# ------------------------------------------------ Murnaghan-Tait water EOS ----
+def _water_cfg(**kw):
+    base = dict(nx=5, ny=4, nz=3, nbasis=4, iquad=2, nQ=5, ieos=2, riemann="hllc", lx=1.0, ly=0.8, lz=0.6,
+                T_floor=1e-4, limiter=False, eos_p1=2.0, eos_pbase=1.0)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+def test_water_eos_point_flux(oracle_mod):
+    """flux_calc_pnts_r with ieos 2: the pressure is P_1(rho^7.2 - 1) + P_base
+    on top of the ideal-gas pressure of the energy (then floored)."""
+    cfg = _water_cfg()
+    o = oracle_mod.Oracle(cfg.as_ints())
+    rng = np.random.default_rng(2)
+    U = np.zeros((20, 5))
+    U[:, 0] = rng.uniform(0.9, 1.1, 20)
+    U[:, 1:4] = rng.uniform(-0.3, 0.3, (20, 3))
+    U[:, 4] = rng.uniform(2.0, 3.0, 20)
+    for d in range(3):
+        F = o.flux_points(U, d + 1)
+        rho, m = U[:, 0], U[:, 1:4]
+        p = (cfg.gamma - 1) * (U[:, 4] - 0.5 * (m ** 2).sum(1) / rho)
+        p = 2.0 * (rho ** 7.2 - 1) + 1.0 + p
+        np.testing.assert_allclose(F[:, 1 + d], m[:, d] ** 2 / rho + p, rtol=1e-13)
+        np.testing.assert_allclose(F[:, 4], (U[:, 4] + p) * m[:, d] / rho, rtol=1e-13)
+
+
+def test_water_eos_hllc_consistency_and_stale_density(oracle_mod):
+    """HLLC with ieos 2 is consistent (equal states give the physical flux),
+    and, as in the reference, every point's wave speeds use 1/rho of the
+    face's last right state: changing that density alone changes the flux at
+    the other points, with a nonzero contact speed there."""
+    cfg = _water_cfg()
+    o = oracle_mod.Oracle(cfg.as_ints())
+    nf = 4
+    rng = np.random.default_rng(7)
+    Q = np.zeros((nf, 5))
+    Q[:, 0] = rng.uniform(0.95, 1.05, nf)
+    Q[:, 1:4] = rng.uniform(-0.2, 0.2, (nf, 3))
+    Q[:, 4] = rng.uniform(2.0, 3.0, nf)
+    for d in range(3):
+        out = o.riemann_face(np.vstack([Q, Q]), d + 1, 1)
+        np.testing.assert_allclose(out, o.flux_points(Q, d + 1), rtol=1e-13, atol=1e-14)
+    QL = Q.copy()
+    QR = Q.copy()
+    QR[:, 0] *= 1.02
+    QR[:, 1] += 0.05
+    base = o.riemann_face(np.vstack([QL, QR]), 1, 1)
+    QR2 = QR.copy()
+    QR2[-1, 0] *= 1.5
+    moved = o.riemann_face(np.vstack([QL, QR2]), 1, 1)
+    assert np.abs(moved[0] - base[0]).max() > 1e-8
+
+
+def test_water_eos_min_dt_closed_form(oracle_mod):
+    """get_min_dt with ieos 2: cs = sqrt(7.2 P_1 rho^6.2 + T_floor)
+    (src/hermeshd.f90:166), plus the last cell's cs, over dxi."""
+    cfg = _water_cfg()
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.1, -0.2, 0.05))
+    rng = np.random.default_rng(3)
+    q[0, 0] *= 1 + 0.01 * rng.standard_normal(q[0, 0].shape)
+    rho = q[0, 0]
+    v = q[0, 1:4] / rho
+    cs = np.sqrt(7.2 * 2.0 * rho ** 6.2 + cfg.T_floor)
+    vmag = (np.abs(v).max(axis=0) + cs).max()
+    want = cfg.cflm / ((vmag + cs.ravel()[-1]) * cfg.dxi)
+    assert abs(o.min_dt(q) - want) <= 1e-14 * want
+
+
+def test_water_eos_conserves_and_keeps_free_stream(oracle_mod):
+    cfg = _water_cfg()
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.2, -0.1, 0.05))
+    assert np.abs(o.rhs(q)).max() < 1e-12
+    rng = np.random.default_rng(4)
+    q[1:] = 0.01 * rng.standard_normal(q[1:].shape)
+    q[0, 0] *= 1 + 0.01 * rng.standard_normal(q[0, 0].shape)
+    r = o.rhs(q)
+    assert np.abs(r[0].reshape(5, -1).sum(axis=1)).max() < 1e-11 * np.abs(r[0]).max()
+    with pytest.raises(ValueError):
+        _water_cfg(riemann="roe")
