@@ -17,7 +17,7 @@ HD_ERR_UNSUPPORTED = -4
 # every extern "C" symbol declared in include/hermeshd_amd.h
 EXPORTED = [
     "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_update",
-    "hd_step", "hd_stage", "hd_rhs_vjp", "hd_stage_vjp", "hd_set_step", "hd_set_inflow", "hd_set_mask", "hd_apply_mask", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
+    "hd_step", "hd_stage", "hd_rhs_vjp", "hd_stage_vjp", "hd_set_step", "hd_set_inflow", "hd_set_mask", "hd_apply_mask", "hd_sample", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
     "hd_last_error",
     "hd_ens_create", "hd_ens_destroy", "hd_ens_state_size", "hd_ens_offsets", "hd_ens_device_bytes",
     "hd_ens_limit", "hd_ens_rhs", "hd_ens_update", "hd_ens_run", "hd_ens_set_step", "hd_ens_check",
@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
     L.hd_set_inflow.argtypes = [vp, vp, vp]
     L.hd_set_mask.argtypes = [vp, vp, C.c_double, vp]
     L.hd_apply_mask.argtypes = [vp, vp, vp]
+    L.hd_sample.argtypes = [vp, vp, C.c_int32, C.c_uint32, dp, vp, vp]
     L.hd_stage.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
     L.hd_rhs_vjp.argtypes = [vp, vp, vp, vp, vp]
     L.hd_stage_vjp.argtypes = [vp, vp, vp, vp, vp, C.POINTER(StageVjpArgs), vp]

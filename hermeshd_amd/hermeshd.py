@@ -33,18 +33,19 @@ from ._lib import NonFiniteError  # noqa: F401  (re-exported)
 from .checkpoint import read_checkpoint, write_checkpoint
 from .config import HDConfig
 from .ics import setup_problem
+from .output import output_vtk
 from .solver import Solver
 
 _input = {"cfg": None, "icid": None, "tf": None, "ntout": 200, "clt": 2.0, "check_every": 1,
           "iread": 0, "iwrite": 0, "fpre": "Qout", "data_dir": "data", "resuming": True, "mpi": (1, 1, 1),
-          "vis": None, "cylinder": False, "inflow": None}
+          "vis": None, "cylinder": False, "inflow": None, "vtk": (), "outdir": "output", "nvtk": 1}
 _state = {"solver": None, "q1_dev": None, "steps": 0}
 
 
 def configure(cfg: HDConfig, icid: int | None = None, tf: float | None = None, ntout: int = 200,
               clt: float = 2.0, check_every: int = 1, iread: int = 0, iwrite: int = 0, fpre: str = "Qout",
               data_dir: str = "data", resuming: bool = True, mpi=(1, 1, 1), vis: float | None = None,
-              cylinder: bool = False, inflow=None) -> None:
+              cylinder: bool = False, inflow=None, vtk=(), outdir: str = "output", nvtk: int = 1) -> None:
     """Set the run's switches (the reference's input.f90 module).  iread > 0
     starts from checkpoint dump iread (keeping its t, dt, nout when resuming);
     iwrite = 1 writes a checkpoint at every output time, alternating dumps 1
@@ -53,10 +54,14 @@ def configure(cfg: HDConfig, icid: int | None = None, tf: float | None = None, n
     vis: the viscosity from which icid 0, 5, 6, 7 set coll and colvis (as
     set_ic does); cylinder: apply the cylinder mask of icid 6, 7 after every
     step; inflow: the x-low inflow state (nQ, nface, nz, ny) when cfg.bc[0] is
-    "inflow" and the initial condition does not define one."""
+    "inflow" and the initial condition does not define one; vtk: the output
+    switches of src/input.f90:87-96 (e.g. ("nstdout", "nstvout")) written as
+    <outdir>/perseus_p0000_t<nout>.vtr at every output time, sampled at
+    nvtk^3 points per cell."""
     _input.update(cfg=cfg, icid=icid, tf=tf, ntout=ntout, clt=clt, check_every=max(0, int(check_every)),
                   iread=int(iread), iwrite=int(iwrite), fpre=fpre, data_dir=data_dir, resuming=bool(resuming),
-                  mpi=tuple(mpi), vis=vis, cylinder=bool(cylinder), inflow=inflow)
+                  mpi=tuple(mpi), vis=vis, cylinder=bool(cylinder), inflow=inflow, vtk=tuple(vtk), outdir=outdir,
+                  nvtk=int(nvtk))
 
 
 def _cfg() -> HDConfig:
@@ -179,13 +184,17 @@ class Hermeshd:
     @staticmethod
     def generate_output(q_r, t, dt, t1, dtout, nout):
         """Output scheduling of generate_output (src/hermeshd.f90:206-266):
-        advances nout when t passes dtout*nout and, with iwrite = 1, writes a
-        checkpoint in the reference's format.  VTK output is not part of this
-        package."""
+        advances nout when t passes dtout*nout, writes the configured VTK
+        fields (output_vtk) and, with iwrite = 1, a checkpoint in the
+        reference's format."""
         tv, dto, n = _get_scalar(t), _get_scalar(dtout), int(np.asarray(nout)) if not isinstance(nout, torch.Tensor) else int(nout.item())
         if dto > 0 and tv > dto * n:
             _set_scalar(nout, n + 1)
             _set_scalar(t1, time.time())
+            if _input["vtk"]:
+                s: Solver = _state["solver"]
+                qd, _ = _as_device(q_r, s.cfg)
+                output_vtk(s, qd, n + 1, 0, _input["outdir"], _input["vtk"], _input["nvtk"])
             if _input["iwrite"] == 1:
                 # writeQ with an odd/even dump id, so the last two sets are kept
                 ioe = 2 - (n + 1) % 2

@@ -174,6 +174,33 @@ int hd_set_mask(hd_ctx *ctx, const uint8_t *mask_dev, double rho_mask, void *str
 /* Applies the mask to q now (no-op without a mask). */
 int hd_apply_mask(hd_ctx *ctx, double *q, void *stream);
 
+/* Derived fields of output_vtk (src/output.f90:14-318) sampled at nvtk^3
+   interior points of every cell (set_vtk_vals_3D, src/basis_funcs.f90:90-200;
+   nvtk 1..4).  `fields` selects bits of HD_OUT_*; out (device, float32)
+   receives one (nz*nvtk, ny*nvtk, nx*nvtk) plane per selected field, x fastest,
+   in bit order.  scales (host, HD_OUT_COUNT entries, or NULL for 1) are the
+   unit factors the reference multiplies by (n0, v0, te0/eV_per_K, 1/t0, ...);
+   the log-density factor is applied inside the logarithm, as there.  The
+   internal energy density is written as computed (the reference computes it
+   and then writes its previous field's buffer). */
+#define HD_OUT_DENSITY 0
+#define HD_OUT_LOG_DENSITY 1
+#define HD_OUT_VX 2
+#define HD_OUT_VY 3
+#define HD_OUT_VZ 4
+#define HD_OUT_TEMPERATURE 5
+#define HD_OUT_INTERNAL_ENERGY 6
+#define HD_OUT_TOTAL_ENERGY 7
+#define HD_OUT_ENTROPY 8
+#define HD_OUT_PRESSURE 9
+#define HD_OUT_PXX 10
+#define HD_OUT_PYY 11
+#define HD_OUT_PZZ 12
+#define HD_OUT_VORTICITY 13
+#define HD_OUT_COUNT 14
+int hd_sample(hd_ctx *ctx, const double *q, int32_t nvtk, uint32_t fields, const double *scales, float *out,
+              void *stream);
+
 /* Waits for the context's work on `stream` and reports the NaN flag raised by
    any stage since the last check (HD_ERR_NONFINITE, with the first bad cell
    index in *bad_cell), then clears it.  The reference exits the process at

@@ -732,6 +732,92 @@ int orc_limiter(orc_ctx *c, double *q) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* output_vtk (src/output.f90:14-318) with set_vtk_vals_3D                   */
+/* (src/basis_funcs.f90:90-200)                                              */
+/* ------------------------------------------------------------------------ */
+int orc_sample(const orc_ctx *c, const double *q, int nvtk, uint32_t fields, const double *scales, float *out) {
+    const orc_config *g = &c->c;
+    const int nb = c->nb, nQ = c->nQ, nv3 = nvtk * nvtk * nvtk;
+    if (nvtk < 1 || nvtk > 4) return set_err(-1, "nvtk must be 1..4");
+    double xg[4];
+    for (int i = 0; i < nvtk; i++) xg[i] = (double)(2 * i + 1 - nvtk) / nvtk;
+    /* bfvtk, bfvtk_dx, bfvtk_dy: sub-point igrid = nvtk^2*(ib) + nvtk*(jb) + kb */
+    double bf[64][NBMAX], bdx[64][NBMAX], bdy[64][NBMAX];
+    for (int ib = 0; ib < nvtk; ib++)
+        for (int jb = 0; jb < nvtk; jb++)
+            for (int kb = 0; kb < nvtk; kb++) {
+                int ig = nvtk * nvtk * ib + nvtk * jb + kb;
+                double *v = bf[ig];
+                double x = xg[ib], y = xg[jb], z = xg[kb];
+                for (int r = 0; r < NBMAX; r++) v[r] = bdx[ig][r] = bdy[ig][r] = 0.;
+                v[K1] = 1.; v[KX] = x; v[KY] = y; v[KZ] = z;
+                fill_products(v, nb);
+                bdx[ig][KX] = 1.; bdy[ig][KY] = 1.;
+                if (nb > 4) {
+                    bdx[ig][KZX] = z; bdx[ig][KXY] = y;
+                    bdy[ig][KYZ] = z; bdy[ig][KXY] = x;
+                }
+                if (nb == 8) { bdx[ig][KXYZ] = y * z; bdy[ig][KXYZ] = z * x; }
+                if (nb == 10) { bdx[ig][KXX] = 3. * x; bdy[ig][KYY] = 3. * y; }
+            }
+    const int nnx = g->nx * nvtk, nny = g->ny * nvtk, nnz = g->nz * nvtk;
+    const long npts = (long)nnx * nny * nnz;
+    int sel[14], nsel = 0;
+    for (int f = 0; f < 14; f++)
+        if (fields & (1u << f)) sel[nsel++] = f;
+    const double gm1 = c->aindex - 1.;
+    for (int k = 0; k < nnz; k++)
+        for (int j = 0; j < nny; j++)
+            for (int i = 0; i < nnx; i++) {
+                int ir = i / nvtk, jr = j / nvtk, kr = k / nvtk;
+                int ig = nvtk * nvtk * (i % nvtk) + nvtk * (j % nvtk) + (k % nvtk);
+                double v[NQMAX];
+                for (int f = 0; f < nQ; f++) {
+                    double s = 0.;
+                    for (int r = 0; r < nb; r++) s += bf[ig][r] * q[qidx(c, ir, jr, kr, f, r)];
+                    v[f] = s;
+                }
+                double dxrh = 0., dyrh = 0., dxmy = 0., dymx = 0.;
+                for (int r = 0; r < nb; r++) {
+                    dxrh += bdx[ig][r] * q[qidx(c, ir, jr, kr, RH, r)];
+                    dyrh += bdy[ig][r] * q[qidx(c, ir, jr, kr, RH, r)];
+                    dxmy += bdx[ig][r] * q[qidx(c, ir, jr, kr, MY, r)];
+                    dymx += bdy[ig][r] * q[qidx(c, ir, jr, kr, MX, r)];
+                }
+                double dxvy = (v[RH] * dxmy - v[MY] * dxrh) / (v[RH] * v[RH]);
+                double dyvx = (v[RH] * dymx - v[MX] * dyrh) / (v[RH] * v[RH]);
+                double dn = v[RH], dni = 1. / dn;
+                double vx = v[MX] * dni, vy = v[MY] * dni, vz = v[MZ] * dni;
+                double U = v[EN] - 0.5 * dn * (vx * vx + vy * vy + vz * vz);
+                double P = gm1 * U;
+                long id = i + (long)nnx * (j + (long)nny * k);
+                for (int s = 0; s < nsel; s++) {
+                    int f = sel[s];
+                    double sc = scales ? scales[f] : 1.0, val = 0.;
+                    switch (f) {
+                    case 0: val = dn * sc; break;
+                    case 1: val = log(dn * sc) / log(10.); break;
+                    case 2: val = sc * v[MX] / v[RH]; break;
+                    case 3: val = sc * v[MY] / v[RH]; break;
+                    case 4: val = sc * v[MZ] / v[RH]; break;
+                    case 5: val = P * sc * dni; break;
+                    case 6: val = U * sc; break;
+                    case 7: val = v[EN] * sc; break;
+                    case 8: val = P * pow(dni, gm1) * sc; break;
+                    case 9: val = P * sc; break;
+                    case 10: val = nQ > PXX ? v[PXX] * sc : 0.; break;
+                    case 11: val = nQ > PYY ? v[PYY] * sc : 0.; break;
+                    case 12: val = nQ > PZZ ? v[PZZ] * sc : 0.; break;
+                    case 13: val = -2. * (dxvy - dyvx) * c->dxi * sc; break;
+                    }
+                    out[(long)s * npts + id] = (float)val;
+                }
+            }
+    (void)nv3;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* exchange_flux + apply_boundaries (src/prepare_step.f90:20-179,            */
 /* src/boundary.f90:40-338,748-782), one device, periodic wrap local         */
 /* ------------------------------------------------------------------------ */

@@ -54,6 +54,11 @@ int orc_set_inflow(orc_ctx *c, const double *def);
    orc_update applies it after every step (orc_apply_mask) */
 int orc_set_mask(orc_ctx *c, const uint8_t *mask, double rho_mask);
 int orc_apply_mask(orc_ctx *c, double *q);
+/* output_vtk's derived fields at nvtk^3 points per cell: out has one
+   (nz*nvtk, ny*nvtk, nx*nvtk) plane per selected field (bits of `fields`,
+   order density, log density, vx, vy, vz, temperature, internal energy,
+   total energy, entropy, pressure, pxx, pyy, pzz, vorticity) */
+int orc_sample(const orc_ctx *c, const double *q, int nvtk, uint32_t fields, const double *scales, float *out);
 
 /* tables (for pinning tests); arrays sized by the caller */
 int orc_tables(const orc_ctx *c, double *xq, double *w1, double *w2, double *w3,

@@ -76,6 +76,7 @@ def lib():
         L.orc_set_inflow.argtypes = [C.c_void_p, dp]
         L.orc_set_mask.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
         L.orc_apply_mask.argtypes = [C.c_void_p, dp]
+        L.orc_sample.argtypes = [C.c_void_p, dp, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_bad_cell.restype = C.c_int64
         _lib = L
@@ -171,6 +172,16 @@ class Oracle:
 
     def apply_mask(self, q):
         self._check(lib().orc_apply_mask(self.h, _p(q)))
+
+    def sample(self, q, nvtk: int, fields: int, scales=None) -> np.ndarray:
+        """output_vtk's fields: (nsel, nz*nvtk, ny*nvtk, nx*nvtk) float32."""
+        nsel = bin(int(fields)).count("1")
+        out = np.zeros((nsel, self.shape[2] * nvtk, self.shape[3] * nvtk, self.shape[4] * nvtk), dtype=np.float32)
+        sc = None if scales is None else np.ascontiguousarray(scales, dtype=np.float64)
+        self._check(lib().orc_sample(self.h, _p(q), int(nvtk), int(fields),
+                                     None if sc is None else sc.ctypes.data_as(C.c_void_p),
+                                     out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def noise_stress(self, stream: int, ids, amp_eta: float, amp_zeta: float) -> np.ndarray:
         """(len(ids), 6) stresses xx,yy,zz,xy,xz,yz at the current step index."""

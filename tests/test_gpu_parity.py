@@ -6,6 +6,8 @@ stress tensor, each over the largest magnitude of that quantity -- for single
 operator calls and after N steps.  The product path runs in FP64 like the
 oracle; differences come from FMA contraction and summation order only.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -479,3 +481,49 @@ def test_front_end_reference_problems(torch_mod, oracle_mod):
             o.step(q, qq1, qq2, tt, dd)
         H.hermeshd.cleanup(t_start)
         assert field_rel_err(np.ascontiguousarray(q_io.T), q) < TOL
+
+
+@pytest.mark.parametrize("name", ["vortex_nb10_hllc", "nb8_roe_walls", "relax_nq11_bodyforce"])
+@pytest.mark.parametrize("nvtk", [1, 3])
+def test_sample_parity(torch_mod, oracle_mod, name, nvtk):
+    """hd_sample (output_vtk's fields) against the oracle: float32 outputs
+    within two float32 ulps (both round the same FP64 values)."""
+    from hermeshd_amd.output import FIELDS, field_mask, sample, unit_scales
+    from hermeshd_amd.solver import Solver
+    cfg = CASES[name]
+    print(f"[sample_parity] {name} nvtk {nvtk}", flush=True)
+    q = smooth_state(cfg, seed=6)
+    s = Solver(cfg)
+    sc = unit_scales()
+    got = sample(s, _dev(q), FIELDS, nvtk=nvtk, scales=sc)
+    ref = oracle_mod.Oracle(cfg.as_ints()).sample(q, nvtk, field_mask(FIELDS), sc)
+    for k, n in enumerate(FIELDS):
+        g = got[n].cpu().numpy()
+        assert g.shape == ref[k].shape
+        np.testing.assert_allclose(g, ref[k], rtol=2.4e-7, atol=1e-30, err_msg=n)
+
+
+def test_front_end_vtk_output(torch_mod, tmp_path):
+    """configure(vtk=...) writes perseus_p0000_t<nout>.vtr at each output time
+    with the reference's default fields in physical units."""
+    from hermeshd_amd import hermeshd as H
+    from hermeshd_amd.output import read_vtr, reference_units
+    cfg = CASES["sod_nb4_hllc"]
+    H.configure(cfg, icid=3, tf=0.01, ntout=2, vtk=("nstdout", "nstvout", "nsttout", "nstpout"),
+                outdir=str(tmp_path / "vtk"))
+    q_io = np.zeros((cfg.nx, cfg.ny, cfg.nz, cfg.nQ, cfg.nbasis), order="F")
+    q1 = np.zeros_like(q_io); q2 = np.zeros_like(q_io)
+    t, dt, t1, t_start, dtout = (np.array(0.0) for _ in range(5))
+    nout = np.array(0)
+    H.hermeshd.setup(q_io, t, dt, t1, t_start, dtout, nout)
+    while float(t) < 0.01:
+        H.hermeshd.step(q_io, q1, q2, t, dt)
+        H.hermeshd.generate_output(q_io, t, dt, t1, dtout, nout)
+    H.hermeshd.cleanup(t_start)
+    files = sorted(os.listdir(tmp_path / "vtk"))
+    assert files and files[0] == "perseus_p0000_t0001.vtr", files
+    d = read_vtr(str(tmp_path / "vtk" / files[-1]))
+    n0 = reference_units()[0]
+    rho = d["Density"] / n0
+    assert d["Velocity"].shape == (cfg.nx, 3) and np.all((rho > 0.1) & (rho < 1.01))
+    assert set(d) >= {"Density", "Velocity", "Temperature", "Pressure", "X", "Y", "Z"}

@@ -785,3 +785,78 @@ def test_water_eos_conserves_and_keeps_free_stream(oracle_mod):
     assert np.abs(r[0].reshape(5, -1).sum(axis=1)).max() < 1e-11 * np.abs(r[0]).max()
     with pytest.raises(ValueError):
         _water_cfg(riemann="roe")
+
+
+# --------------------------------------------------------- field output ----
+def _leg(a, x):
+    return [np.ones_like(x), x, 1.5 * x ** 2 - 0.5][a]
+
+
+@pytest.mark.parametrize("nb,iq", [(4, 2), (8, 2), (10, 3)])
+def test_sample_evaluates_the_modal_expansion(oracle_mod, nb, iq):
+    """output_vtk's point values at the nvtk^3 sub-cell points equal the
+    Legendre expansion evaluated there (closed form per mode), and the
+    derived fields follow from them."""
+    cfg = HDConfig(nx=3, ny=2, nz=2, nbasis=nb, iquad=iq, nQ=11, ivis=1, lx=1.5, ly=1.0, lz=1.0, T_floor=1e-4)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    rng = np.random.default_rng(nb)
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.2, 0.1, -0.1))
+    q[1:] = 0.05 * rng.standard_normal(q[1:].shape)
+    q[0, 5:] = rng.uniform(0.5, 1.0, q[0, 5:].shape)
+    deg = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)]
+    deg += [(1, 1, 1)] if nb == 8 else [(2, 0, 0), (0, 2, 0), (0, 0, 2)] if nb == 10 else []
+    deg = deg[:nb]
+    for nvtk in (1, 2, 3):
+        out = o.sample(q, nvtk, (1 << 14) - 1)
+        pts = (2 * np.arange(nvtk) + 1 - nvtk) / nvtk
+        # reconstruct the expansion on the sub-grid: (f, z', y', x')
+        X = np.tile(pts, cfg.nx)[None, None, :]
+        Y = np.tile(pts, cfg.ny)[None, :, None]
+        Z = np.tile(pts, cfg.nz)[:, None, None]
+        rep = lambda a: np.repeat(np.repeat(np.repeat(a, nvtk, 0), nvtk, 1), nvtk, 2)  # noqa: E731
+        val = np.zeros((11, cfg.nz * nvtk, cfg.ny * nvtk, cfg.nx * nvtk))
+        for r, (a, b, c) in enumerate(deg):
+            phi = _leg(a, X) * _leg(b, Y) * _leg(c, Z)
+            for f in range(11):
+                val[f] += phi * rep(q[r, f])
+        rho = val[0]
+        u = val[1:4] / rho
+        U = val[4] - 0.5 * rho * (u ** 2).sum(0)
+        P = (cfg.gamma - 1) * U
+        want = [rho, np.log10(rho), u[0], u[1], u[2], P / rho, U, val[4], P * rho ** -(cfg.gamma - 1), P,
+                val[5], val[6], val[7]]
+        for k, w in enumerate(want):
+            np.testing.assert_allclose(out[k], w.astype(np.float32), rtol=2e-6, atol=1e-6)
+
+
+def test_sample_vorticity_of_a_shear_flow(oracle_mod):
+    """rho = 1, m_x = s*eta (y slope s), m_y = t*xi: output_vtk writes
+    -2*(dv/dxi - du/deta)*dxi = -2*(t - s)*dxi, i.e. minus the vorticity
+    dv/dx - du/dy when dx = dy."""
+    cfg = HDConfig(nx=4, ny=4, nz=1, nbasis=4, iquad=2, nQ=5, lx=2.0, ly=2.0, lz=0.5, T_floor=1e-4)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q = ics.uniform(cfg, rho=1.0, T=1.0)
+    s, t = 0.3, -0.1
+    q[2, MX] = s   # y slope of m_x
+    q[1, MY] = t   # x slope of m_y
+    vort = o.sample(q, 2, 1 << 13)[0]
+    np.testing.assert_allclose(vort, -2 * (t - s) * cfg.dxi, rtol=1e-6)
+    dudy, dvdx = s * 2 * cfg.ny / cfg.ly, t * 2 * cfg.nx / cfg.lx
+    np.testing.assert_allclose(vort, -(dvdx - dudy), rtol=1e-6)
+
+
+def test_vtr_writer_round_trip(tmp_path):
+    from hermeshd_amd.output import read_vtr, write_vtr
+    cfg = HDConfig(nx=3, ny=2, nz=2, nbasis=4, iquad=2, nQ=5, lx=3.0, ly=2.0, lz=1.0)
+    rng = np.random.default_rng(0)
+    rho = rng.random((4, 4, 6)).astype(np.float32)
+    vel = rng.random((3, 4, 4, 6)).astype(np.float32)
+    path = tmp_path / "out" / "perseus_p0000_t0001.vtr"
+    write_vtr(str(path), cfg, {"Density": rho, "Velocity": vel}, nvtk=2)
+    back = read_vtr(str(path))
+    np.testing.assert_array_equal(back["Density"], rho.ravel())
+    np.testing.assert_array_equal(back["Velocity"], np.stack([vel[c].ravel() for c in range(3)], axis=1))
+    np.testing.assert_allclose(back["X"], np.linspace(-1.5, 1.5, 7), rtol=1e-6)
+    np.testing.assert_allclose(back["Z"], np.linspace(-0.5, 0.5, 5), rtol=1e-6)
+    text = path.read_text()
+    assert 'WholeExtent="0 6 0 4 0 4"' in text
