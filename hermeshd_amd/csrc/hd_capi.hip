@@ -293,6 +293,11 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         hd_destroy(c);
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
+    // workgroup size of the fused stage kernel: 64, 128 or 256 threads
+    if (const char* bs = std::getenv("HD_STAGE_BLOCK")) {
+        const int v = std::atoi(bs);
+        if (v == 64 || v == 128 || v == 256) c->stage_bs = v;
+    }
     const char* kern = std::getenv("HD_KERNELS");
     // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
     // table-driven kernels only

@@ -49,6 +49,10 @@ struct hd_ctx {
     double rho_mask = 0.;
     double kap_f = 0., kap_v = 0.; // point amplitude factors of the face and volume rules
     bool generic = false;          // HD_KERNELS=generic selects the table-driven kernels
+    // threads per workgroup of the fused stage kernel (HD_STAGE_BLOCK): one
+    // wave per workgroup lets each SIMD start its next wave as soon as its
+    // last one ends (the waves' run times differ with the limiter's early-out)
+    int stage_bs = 64;
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage

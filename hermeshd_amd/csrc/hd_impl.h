@@ -25,14 +25,15 @@ struct Impl {
                            c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
     }
     static void fused(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        const int bs = c->stage_bs;
         if constexpr (NQ >= 11) {
             if (sp.noise[0]) {
-                hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, true>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s,
+                hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, true>), dim3(blocks_for(c->N, bs)), dim3(bs), 0, s,
                                    qin, qn, qout, c->grid, c->ph, sp);
                 return;
             }
         }
-        hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, false>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, qin, qn,
+        hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, false>), dim3(blocks_for(c->N, bs)), dim3(bs), 0, s, qin, qn,
                            qout, c->grid, c->ph, sp);
     }
     static void ens_limit(hd_ens* e, double* q, hipStream_t s) {
