@@ -127,6 +127,85 @@ __device__ __forceinline__ const FaceTab<8, 2>& face_tab<8>() { return c_face_8;
 template <>
 __device__ __forceinline__ const FaceTab<10, 3>& face_tab<10>() { return c_face_10; }
 
+// ------------------------------------------------------ face traces ------
+// On a face the basis collapses to a polynomial in the two tangential
+// coordinates (row = the ii coordinate, col = the jj coordinate of the face
+// point order above): phi_r = Lface(normal degree) * L(dr, row) * L(dc, col),
+// with (dr, dc) in {(0,0), (1,0), (0,1), (1,1), (2,0), (0,2)}.  A side's
+// trace is therefore sum_k K_k b_k(p) with NK = 3 (nb 4), 4 (nb 8) or 6
+// (nb 10) coefficients K_k = sum of +-m_r over the modes of pair k: NK
+// products per point instead of NB.  Both cells of a face evaluate a side's
+// trace with this same arithmetic, so they still agree bitwise.
+template <int NB>
+HD_DEV constexpr int face_nk() { return NB == 10 ? 6 : (NB == 8 ? 4 : 3); }
+HD_DEV constexpr int pair_index(int dr, int dc) {
+    return dr == 0 ? (dc == 0 ? 0 : (dc == 1 ? 2 : 5)) : (dr == 1 ? (dc == 0 ? 1 : 3) : 4);
+}
+HD_DEV constexpr int pair_dr(int k) { return k == 1 || k == 3 ? 1 : (k == 4 ? 2 : 0); }
+HD_DEV constexpr int pair_dc(int k) { return k == 2 || k == 3 ? 1 : (k == 5 ? 2 : 0); }
+// tangential axes of a face: row = ii coordinate, col = jj coordinate
+HD_DEV constexpr int row_axis(int dir) { return dir == 0 ? 2 : (dir == 1 ? 0 : 1); }
+HD_DEV constexpr int col_axis(int dir) { return dir == 0 ? 1 : (dir == 1 ? 2 : 0); }
+
+// b_k at face point p = (ii, jj), jj fastest
+template <int NB, int QP>
+struct FacePoly {
+    double b[QP * QP][face_nk<NB>()];
+};
+template <int NB, int QP>
+constexpr FacePoly<NB, QP> make_face_poly() {
+    FacePoly<NB, QP> t{};
+    for (int ii = 0; ii < QP; ii++)
+        for (int jj = 0; jj < QP; jj++)
+            for (int k = 0; k < face_nk<NB>(); k++)
+                t.b[ii * QP + jj][k] = Lval<QP>(pair_dr(k), ii) * Lval<QP>(pair_dc(k), jj);
+    return t;
+}
+static __constant__ FacePoly<4, 2> c_fpoly_4 = make_face_poly<4, 2>();
+static __constant__ FacePoly<8, 2> c_fpoly_8 = make_face_poly<8, 2>();
+static __constant__ FacePoly<10, 3> c_fpoly_10 = make_face_poly<10, 3>();
+template <int NB>
+__device__ __forceinline__ const FacePoly<NB, (NB == 10 ? 3 : 2)>& face_poly();
+template <>
+__device__ __forceinline__ const FacePoly<4, 2>& face_poly<4>() { return c_fpoly_4; }
+template <>
+__device__ __forceinline__ const FacePoly<8, 2>& face_poly<8>() { return c_fpoly_8; }
+template <>
+__device__ __forceinline__ const FacePoly<10, 3>& face_poly<10>() { return c_fpoly_10; }
+
+// K[f][k] of the side SIDE (0: coordinate -1, 1: +1) of a DIR face of a cell with modes m
+template <int NB, int NQ, int DIR, int SIDE, int NK>
+__device__ __forceinline__ void face_coeffs(const double (&m)[NQ][NB], double (&K)[NQ][NK]) {
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            double s = 0.;
+            bool first = true;
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                if (pair_index(deg<NB>(r, row_axis(DIR)), deg<NB>(r, col_axis(DIR))) != k) continue;
+                const bool neg = Lface(deg<NB>(r, DIR), SIDE) < 0.0;
+                if (first) s = neg ? -m[f][r] : m[f][r];
+                else s = neg ? s - m[f][r] : s + m[f][r];
+                first = false;
+            }
+            K[f][k] = s;
+        }
+    }
+}
+
+template <int NQ, int NK>
+__device__ __forceinline__ void face_eval(const double (&K)[NQ][NK], const double* __restrict__ b, double* out) {
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        double s = K[f][0];
+#pragma unroll
+        for (int k = 1; k < NK; k++) s += K[f][k] * b[k];
+        out[f] = s;
+    }
+}
+
 // Keeps the scheduler from interleaving fully unrolled quadrature points,
 // which would stretch live ranges past the register file.
 __device__ __forceinline__ void point_barrier() { __builtin_amdgcn_sched_barrier(0); }
@@ -229,7 +308,10 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
     const bool at_edge = SIDE ? (a == n_ax - 1) : (a == 0);
     const int bc = g.bc[2 * DIR + SIDE];
     const bool ext = at_edge && bc != BC_PERIODIC;
-    double nm[NQ][NB];
+    constexpr int NK = face_nk<NB>();
+    const auto& FP = face_poly<NB>();
+    double ko[NQ][NK], kn[NQ][NK];
+    face_coeffs<NB, NQ, DIR, SIDE, NK>(m, ko);
     if (!ext) {
         int an = SIDE ? a + 1 : a - 1;
         if (an < 0) an += n_ax;
@@ -237,7 +319,9 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
         const long cn = DIR == 0 ? (cp.c - cp.i + an)
                       : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
                                  : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
+        double nm[NQ][NB];
         load_modes<NB, NQ>(q, g, cn, nm);
+        face_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nm, kn);
     }
     // outflow boundaries decide on the whole face (src/boundary.f90:213-338)
     bool zero_normal = false;
@@ -262,21 +346,9 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
 #pragma unroll 1
     for (int p = 0; p < NF; p++) {
         double own[NQ], oth[NQ];
-#pragma unroll
-        for (int f = 0; f < NQ; f++) {
-            double s = 0.;
-#pragma unroll
-            for (int r = 0; r < NB; r++) s += T.bf[DIR][SIDE][p][r] * m[f][r];
-            own[f] = s;
-        }
+        face_eval<NQ, NK>(ko, FP.b[p], own);
         if (!ext) {
-#pragma unroll
-            for (int f = 0; f < NQ; f++) {
-                double s = 0.;
-#pragma unroll
-                for (int r = 0; r < NB; r++) s += T.bf[DIR][1 - SIDE][p][r] * nm[f][r];
-                oth[f] = s;
-            }
+            face_eval<NQ, NK>(kn, FP.b[p], oth);
         } else {
 #pragma unroll
             for (int f = 0; f < NQ; f++) oth[f] = own[f];
