@@ -527,3 +527,30 @@ def test_front_end_vtk_output(torch_mod, tmp_path):
     rho = d["Density"] / n0
     assert d["Velocity"].shape == (cfg.nx, 3) and np.all((rho > 0.1) & (rho < 1.01))
     assert set(d) >= {"Density", "Velocity", "Temperature", "Pressure", "X", "Y", "Z"}
+
+
+@pytest.mark.parametrize("name", ["vortex_nb10_hllc", "relax_nq11_bodyforce"])
+def test_stage_workgroup_size_does_not_change_results(torch_mod, name):
+    """The fused stage kernel computes each cell independently: 64-, 128- and
+    256-thread workgroups (HD_STAGE_BLOCK) give bitwise the same steps."""
+    from hermeshd_amd.solver import Solver
+    cfg = CASES[name]
+    q0 = _dev(smooth_state(cfg, seed=8))
+    outs = []
+    for bs in ("64", "128", "256"):
+        old = os.environ.get("HD_STAGE_BLOCK")
+        os.environ["HD_STAGE_BLOCK"] = bs
+        try:
+            s = Solver(cfg)
+        finally:
+            if old is None:
+                del os.environ["HD_STAGE_BLOCK"]
+            else:
+                os.environ["HD_STAGE_BLOCK"] = old
+        q = q0.clone()
+        q1 = torch_mod.zeros_like(q)
+        for _ in range(3):
+            s.step(q, q1)
+        s.check()
+        outs.append(q)
+    assert torch_mod.equal(outs[0], outs[1]) and torch_mod.equal(outs[0], outs[2])
