@@ -374,6 +374,29 @@ __device__ __forceinline__ void load_lp(const double* __restrict__ lam, const Gr
         }
 }
 
+// Lifted cotangent of a face side in face-polynomial form: the transposed
+// lift sum_r cf[side][p][r] l_r equals sum_k wb_k(p) L_k with
+// L_k = sum over the modes r of pair k of (+-0.25 cb_r Lface_r) l_r.
+template <int NB, int NQ, int DIR, int SIDE, int NK>
+__device__ __forceinline__ void lift_coeffs(const double (&l)[NQ][NB], double (&L)[NQ][NK]) {
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            double s = 0.;
+            bool first = true;
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                if (fz::pair_index(fz::deg<NB>(r, fz::row_axis(DIR)), fz::deg<NB>(r, fz::col_axis(DIR))) != k) continue;
+                constexpr double q = SIDE ? -0.25 : 0.25;
+                const double c = q * fz::cbasis<NB>(r) * fz::Lface(fz::deg<NB>(r, DIR), SIDE);
+                s = first ? c * l[f][r] : s + c * l[f][r];
+                first = false;
+            }
+            L[f][k] = s;
+        }
+}
+
 // one face of the cell: adds this cell's share of (dR/dm)^T lp to rt
 template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
 __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, const double* __restrict__ lam,
@@ -388,7 +411,15 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
     const bool at_edge = SIDE ? (a == n_ax - 1) : (a == 0);
     const int bc = g.bc[2 * DIR + SIDE];
     const bool ext = at_edge && bc != BC_PERIODIC;
-    double nm[NQ][NB], nl[NQ][NB];
+    // the neighbour enters through its face polynomial: its trace
+    // sum_k kn_k b_k(p) and its lifted cotangent sum_k ln_k wb_k(p), so its
+    // modes and cotangents are collapsed as they are loaded
+    constexpr int NK = fz::face_nk<NB>();
+    const auto& FP = fz::face_poly<NB>();
+    double ko[NQ][NK], kn[NQ][NK], ln[NQ][NK];
+    // both sides' traces with the forward kernel's arithmetic, so that every
+    // Riemann branch is taken as in the forward stage
+    fz::face_coeffs<NB, NQ, DIR, SIDE, NK>(m, ko);
     if (!ext) {
         int an = SIDE ? a + 1 : a - 1;
         if (an < 0) an += n_ax;
@@ -396,8 +427,11 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
         const long cn = DIR == 0 ? (cp.c - cp.i + an)
                       : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
                                  : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
+        double nm[NQ][NB], nl[NQ][NB];
         fz::load_modes<NB, NQ>(q, g, cn, nm);
+        fz::face_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nm, kn);
         load_lp<NB, NQ>(lam, g, cn, scale, faci, nl);
+        lift_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nl, ln);
     }
     bool zero_normal = false;
     if (ext && bc == BC_OUTFLOW && !(DIR == 0 && SIDE == 1)) {
@@ -413,28 +447,22 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
 #pragma unroll 1
     for (int p = 0; p < NF; p++) {
         double own[NQ], oth[NQ], mu[NQ];
+        fz::face_eval<NQ, NK>(ko, FP.b[p], own);
 #pragma unroll
         for (int f = 0; f < NQ; f++) {
-            double s = 0., t = 0.;
+            double t = 0.;
 #pragma unroll
-            for (int r = 0; r < NB; r++) {
-                s += T.bf[DIR][SIDE][p][r] * m[f][r];
-                t += T.cf[DIR][SIDE][p][r] * lo[f][r];
-            }
-            own[f] = s;
+            for (int r = 0; r < NB; r++) t += T.cf[DIR][SIDE][p][r] * lo[f][r];
             mu[f] = t;
         }
         bool flip_mx = false;
         if (!ext) {
+            fz::face_eval<NQ, NK>(kn, FP.b[p], oth);
 #pragma unroll
             for (int f = 0; f < NQ; f++) {
-                double s = 0., t = 0.;
+                double t = ln[f][0] * FP.wb[p][0];
 #pragma unroll
-                for (int r = 0; r < NB; r++) {
-                    s += T.bf[DIR][1 - SIDE][p][r] * nm[f][r];
-                    t += T.cf[DIR][1 - SIDE][p][r] * nl[f][r];
-                }
-                oth[f] = s;
+                for (int k = 1; k < NK; k++) t += ln[f][k] * FP.wb[p][k];
                 mu[f] += t;
             }
         } else {

@@ -147,18 +147,21 @@ HD_DEV constexpr int pair_dc(int k) { return k == 2 || k == 3 ? 1 : (k == 5 ? 2 
 HD_DEV constexpr int row_axis(int dir) { return dir == 0 ? 2 : (dir == 1 ? 0 : 1); }
 HD_DEV constexpr int col_axis(int dir) { return dir == 0 ? 1 : (dir == 1 ? 2 : 0); }
 
-// b_k at face point p = (ii, jj), jj fastest
+// b_k at face point p = (ii, jj), jj fastest, and w2_p*b_k (lifts)
 template <int NB, int QP>
 struct FacePoly {
     double b[QP * QP][face_nk<NB>()];
+    double wb[QP * QP][face_nk<NB>()];
 };
 template <int NB, int QP>
 constexpr FacePoly<NB, QP> make_face_poly() {
     FacePoly<NB, QP> t{};
     for (int ii = 0; ii < QP; ii++)
         for (int jj = 0; jj < QP; jj++)
-            for (int k = 0; k < face_nk<NB>(); k++)
+            for (int k = 0; k < face_nk<NB>(); k++) {
                 t.b[ii * QP + jj][k] = Lval<QP>(pair_dr(k), ii) * Lval<QP>(pair_dc(k), jj);
+                t.wb[ii * QP + jj][k] = Rule<QP>::w2(ii, jj) * t.b[ii * QP + jj][k];
+            }
     return t;
 }
 static __constant__ FacePoly<4, 2> c_fpoly_4 = make_face_poly<4, 2>();
