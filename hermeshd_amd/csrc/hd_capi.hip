@@ -298,6 +298,13 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         const int v = std::atoi(bs);
         if (v == 64 || v == 128 || v == 256) c->stage_bs = v;
     }
+    {
+        const size_t bytes = (size_t)c->nface * c->nQ * cfg->nx * sizeof(double);
+        const char* xs = std::getenv("HD_STAGE_XS");
+        const bool off = xs && std::strcmp(xs, "0") == 0;
+        if (!off && cfg->bc[0] == HD_BC_PERIODIC && cfg->nx % 64 == 0 && cfg->nx <= 256 && bytes <= 65536)
+            c->xs_bytes = (unsigned)bytes;
+    }
     const char* kern = std::getenv("HD_KERNELS");
     // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
     // table-driven kernels only

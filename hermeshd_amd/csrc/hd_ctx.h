@@ -53,6 +53,10 @@ struct hd_ctx {
     // wave per workgroup lets each SIMD start its next wave as soon as its
     // last one ends (the waves' run times differ with the limiter's early-out)
     int stage_bs = 64;
+    // LDS bytes of the row-shared x-face stage kernel, 0 when it does not apply
+    // (x not periodic, nx not a multiple of 64 up to 256, or too much LDS);
+    // HD_STAGE_XS=0 switches it off
+    unsigned xs_bytes = 0;
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage

@@ -300,10 +300,12 @@ __device__ __forceinline__ void inflow_ghost(const double* __restrict__ inflow, 
 }
 
 // One face of the cell: DIR 0/1/2, SIDE 0 (minus) / 1 (plus).
-template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE>
+// STORE: also write each point's scaled flux to xs[(p*NQ + f)*nx + i] for the
+// x-neighbour's lift (the row-shared x faces of k_stage_xs).
+template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE, bool STORE = false>
 __device__ __forceinline__ void face_term(const double* __restrict__ q, const Grid& g, const CellPos& cp,
                                           const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
-                                          double dxi_d, const StageParams& sp) {
+                                          double dxi_d, const StageParams& sp, double* xs = nullptr) {
     constexpr int NF = QP * QP;
     const auto& T = face_tab<NB>();
     const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
@@ -399,8 +401,29 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
 #pragma unroll
         for (int f = 0; f < NQ; f++) {
             const double v = dxi_d * out[f];
+            if constexpr (STORE) xs[(long)(p * NQ + f) * g.nx + cp.i] = v;
 #pragma unroll
             for (int r = 0; r < NB; r++) rhs[f][r] += T.cf[DIR][SIDE][p][r] * v;
+        }
+    }
+}
+
+// The -x face of a cell is the +x face of its left neighbour in the same row:
+// its scaled point fluxes come from xs (written by that neighbour), bitwise
+// what the cell would compute itself, and only the lift is done here.
+template <int NB, int QP, int NQ>
+__device__ __forceinline__ void x_lift_shared(const double* __restrict__ xs, const Grid& g, const CellPos& cp,
+                                              double (&rhs)[NQ][NB]) {
+    constexpr int NF = QP * QP;
+    const auto& T = face_tab<NB>();
+    const int il = cp.i == 0 ? g.nx - 1 : cp.i - 1;
+#pragma unroll 1
+    for (int p = 0; p < NF; p++) {
+#pragma unroll
+        for (int f = 0; f < NQ; f++) {
+            const double v = xs[(long)(p * NQ + f) * g.nx + il];
+#pragma unroll
+            for (int r = 0; r < NB; r++) rhs[f][r] += T.cf[0][0][p][r] * v;
         }
     }
 }
@@ -539,10 +562,13 @@ __device__ __forceinline__ void fence() { asm volatile("" ::: "memory"); }
 
 // One cell of one stage; `bad_id` is the index reported when mode 1 of the
 // output is NaN.
-template <int NB, int QP, int NQ, int RM, bool NZ>
+// XS: the workgroup is one periodic row of cells (k_stage_xs); each x face is
+// evaluated once, by its left cell, and shared through xs (LDS).
+template <int NB, int QP, int NQ, int RM, bool NZ, bool XS = false>
 __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const double* __restrict__ qn,
                                            double* __restrict__ qout, const Grid& g, const Phys& ph,
-                                           const StageParams& sp, const CellPos& cp, long bad_id) {
+                                           const StageParams& sp, const CellPos& cp, long bad_id,
+                                           double* xs = nullptr) {
     const long c = cp.c;
     double m[NQ][NB];
     load_modes<NB, NQ>(qin, g, c, m);
@@ -555,9 +581,15 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const
     // hoist all six neighbours' modes at once (which would spill).
     volume_term<NB, QP, NQ, NZ>(m, rhs, ph, g.dxi, g.dyi, g.dzi, sp, c, g.N);
     fence();
-    face_term<NB, QP, NQ, RM, NZ, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi, sp);
-    fence();
-    face_term<NB, QP, NQ, RM, NZ, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi, sp);
+    if constexpr (XS) {
+        face_term<NB, QP, NQ, RM, NZ, 0, 1, true>(qin, g, cp, m, rhs, ph, g.dxi, sp, xs);
+        __syncthreads();
+        x_lift_shared<NB, QP, NQ>(xs, g, cp, rhs);
+    } else {
+        face_term<NB, QP, NQ, RM, NZ, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi, sp);
+        fence();
+        face_term<NB, QP, NQ, RM, NZ, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi, sp);
+    }
     fence();
     face_term<NB, QP, NQ, RM, NZ, 1, 0>(qin, g, cp, m, rhs, ph, g.dyi, sp);
     fence();
@@ -642,6 +674,24 @@ __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, c
     cp.j = (int)((c / g.nx) % g.ny);
     cp.k = (int)(c / ((long)g.nx * g.ny));
     stage_cell<NB, QP, NQ, RM, NZ>(qin, qn, qout, g, ph, sp, cp, c);
+}
+
+// One periodic row of nx cells per workgroup (nx a multiple of 64, at most
+// 256): the x faces are evaluated once and shared through LDS
+// (NF*NQ*nx doubles of dynamic shared memory).  The grid covers the rows
+// exactly, so every thread reaches the barrier.
+template <int NB, int QP, int NQ, int RM, bool NZ>
+__global__ __launch_bounds__(256) void k_stage_xs(const double* __restrict__ qin, const double* __restrict__ qn,
+                                                  double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
+    extern __shared__ double xs[];
+    const long row = (long)xcd_block();
+    const long c = row * g.nx + threadIdx.x;
+    CellPos cp;
+    cp.c = c;
+    cp.i = (int)threadIdx.x;
+    cp.j = (int)(row % g.ny);
+    cp.k = (int)(row / g.ny);
+    stage_cell<NB, QP, NQ, RM, NZ, true>(qin, qn, qout, g, ph, sp, cp, c, xs);
 }
 
 }  // namespace fz

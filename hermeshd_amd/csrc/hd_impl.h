@@ -26,6 +26,20 @@ struct Impl {
     }
     static void fused(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
         const int bs = c->stage_bs;
+        if (c->xs_bytes) {
+            // one periodic row per workgroup, x faces shared through LDS
+            const unsigned rows = (unsigned)(c->N / c->grid.nx);
+            if constexpr (NQ >= 11) {
+                if (sp.noise[0]) {
+                    hipLaunchKernelGGL((fz::k_stage_xs<NB, QP, NQ, RM, true>), dim3(rows), dim3(c->grid.nx),
+                                       c->xs_bytes, s, qin, qn, qout, c->grid, c->ph, sp);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((fz::k_stage_xs<NB, QP, NQ, RM, false>), dim3(rows), dim3(c->grid.nx), c->xs_bytes, s,
+                               qin, qn, qout, c->grid, c->ph, sp);
+            return;
+        }
         if constexpr (NQ >= 11) {
             if (sp.noise[0]) {
                 hipLaunchKernelGGL((fz::k_stage<NB, QP, NQ, RM, true>), dim3(blocks_for(c->N, bs)), dim3(bs), 0, s,

@@ -68,6 +68,9 @@ CASES = {
     "relax_nb10_roe": HDConfig(nx=5, ny=6, nz=4, nbasis=10, iquad=3, nQ=11, ivis=1, riemann="roe", coll=50.0,
                                colvis=0.5, bc=("periodic", "periodic", "noslip", "noslip", "wall", "wall"),
                                lx=1.0, ly=1.2, lz=0.8, T_floor=1e-4, body_force=(1e-2, -2e-3, 0)),
+    # periodic rows of 64 cells: the row-shared x-face stage kernel
+    "vortex_nb10_rows64": vortex_3d(64, 5, 3),
+    "relax_nq11_rows64": poiseuille_2d(64, 6),
     # full 10-moment equations (ivis 2): table-driven kernels
     "tenmom_nb4_hllc_periodic": HDConfig(nx=6, ny=5, nz=4, nbasis=4, iquad=2, nQ=11, ivis=2, coll=5.0,
                                          riemann="hllc", lx=1.2, ly=1.0, lz=0.8, T_floor=1e-4),
@@ -146,7 +149,8 @@ def test_min_dt_parity(torch_mod, oracle_mod, name):
                                          ("nq11_hllc_xoutflow", 4),
                                          ("relax_nb10_roe", 5), ("tenmom_nb4_hllc_periodic", 6),
                                          ("tenmom_nb10_roe_walls", 3), ("tenmom_nb8_central_outflow", 4),
-                                         ("water_nb4_hllc_periodic", 5), ("water_nb10_central_walls", 3)])
+                                         ("water_nb4_hllc_periodic", 5), ("water_nb10_central_walls", 3),
+                                         ("vortex_nb10_rows64", 4), ("relax_nq11_rows64", 6)])
 def test_step_parity(torch_mod, oracle_mod, name, nsteps):
     """Reference step semantics (dt from get_min_dt, RK update, t += dt) for N steps."""
     from hermeshd_amd.solver import Solver
@@ -554,3 +558,31 @@ def test_stage_workgroup_size_does_not_change_results(torch_mod, name):
         s.check()
         outs.append(q)
     assert torch_mod.equal(outs[0], outs[1]) and torch_mod.equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("name", ["vortex_nb10_rows64", "relax_nq11_rows64"])
+def test_row_shared_x_faces_match_per_cell_faces(torch_mod, name):
+    """The row-shared x-face stage kernel (one periodic row per workgroup, each
+    x face evaluated once) against the per-cell kernel (HD_STAGE_XS=0): the
+    same fluxes, lifted in another order, so equal to rounding."""
+    from hermeshd_amd.solver import Solver
+    cfg = CASES[name]
+    q0 = _dev(smooth_state(cfg, seed=9))
+    outs = []
+    for xs in ("1", "0"):
+        old = os.environ.get("HD_STAGE_XS")
+        os.environ["HD_STAGE_XS"] = xs
+        try:
+            s = Solver(cfg)
+        finally:
+            if old is None:
+                del os.environ["HD_STAGE_XS"]
+            else:
+                os.environ["HD_STAGE_XS"] = old
+        q = q0.clone()
+        q1 = torch_mod.zeros_like(q)
+        for _ in range(3):
+            s.step(q, q1)
+        s.check()
+        outs.append(q.cpu().numpy())
+    assert field_rel_err(outs[0], outs[1]) < 1e-13
