@@ -276,8 +276,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         c->kap_f = rule_kappa(c->htab, 2);
         c->kap_v = rule_kappa(c->htab, 3);
         const size_t nfw = (size_t)6 * c->nface * c->nfb, nvw = (size_t)6 * c->htab.npg * c->N;
-        for (int d = 0; d < 3 && r == HD_OK; d++) r = alloc(c, (void**)&c->noise_f[d], nfw * sizeof(double));
-        if (r == HD_OK) r = alloc(c, (void**)&c->noise_v, nvw * sizeof(double));
+        for (int d = 0; d < 3 && r == HD_OK; d++) r = alloc(c, (void**)&c->noise_f[d], nfw * sizeof(float));
+        if (r == HD_OK) r = alloc(c, (void**)&c->noise_v, nvw * sizeof(float));
         if (r) {
             std::string msg = g_last_error;
             hd_destroy(c);

@@ -37,8 +37,8 @@ struct hd_ctx {
     int vjp_blocks = 0;            // blocks of the last transpose launch
     // stochastic stress (cfg.noise): the step's field and the step counter
     unsigned long long* step_ctr = nullptr;
-    double* noise_f[3] = {nullptr, nullptr, nullptr};
-    double* noise_v = nullptr;
+    float* noise_f[3] = {nullptr, nullptr, nullptr};
+    float* noise_v = nullptr;
     long nfb = 0;                  // face lattice size (nx+1)(ny+1)(nz+1)
     // x-low inflow exterior state (ny,nz,nface,nQ), set by hd_set_inflow
     double* inflow = nullptr;
@@ -81,8 +81,8 @@ struct hd_ens {
     long* dpts = nullptr;
     int2* dwork = nullptr;
     size_t work_cap = 0;
-    long state_doubles = 0, cells = 0, noise_doubles = 0;
-    double* noise = nullptr;
+    long state_doubles = 0, cells = 0, noise_floats = 0;
+    float* noise = nullptr;
     double* one_dt = nullptr;          // 1.0: the noise time step of hd_ens_rhs
     double* stage_buf = nullptr;       // RK3 third array when the caller passes none
     unsigned long long* step_ctr = nullptr;

@@ -239,7 +239,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     }
     e->state_doubles = qoff;
     e->cells = cell0;
-    e->noise_doubles = noff;
+    e->noise_floats = noff;
     rc = HD_OK;
     if ((rc = ens_alloc(e, (void**)&e->dreal, nreal * sizeof(ens::EnsReal))) ||
         (rc = ens_alloc(e, (void**)&e->dpts, (nreal + 1) * sizeof(long))) ||
@@ -247,7 +247,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
         (rc = ens_alloc(e, (void**)&e->step_ctr, sizeof(unsigned long long))) ||
         (rc = ens_alloc(e, (void**)&e->err, sizeof(int))) ||
         (rc = ens_alloc(e, (void**)&e->bad, sizeof(unsigned long long))) ||
-        (noff > 0 && (rc = ens_alloc(e, (void**)&e->noise, noff * sizeof(double)))) ||
+        (noff > 0 && (rc = ens_alloc(e, (void**)&e->noise, noff * sizeof(float)))) ||
         (base->integrator == 3 && (rc = ens_alloc(e, (void**)&e->stage_buf, qoff * sizeof(double))))) {
         std::string msg = g_last_error;
         hd_ens_destroy(e);

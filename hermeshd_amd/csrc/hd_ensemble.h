@@ -73,7 +73,7 @@ template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_ens_stage(const double* __restrict__ qin, const double* __restrict__ qn,
                                                    double* __restrict__ qout, const EnsReal* __restrict__ reals,
                                                    const int2* __restrict__ work, Launch L, Phys ph,
-                                                   StageParams sp, const double* __restrict__ noise) {
+                                                   StageParams sp, const float* __restrict__ noise) {
     constexpr int NF = QP * QP;
     for_each_item(L, [&](int w) {
         const int2 wk = work[w];
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_ens_stage(const double* __restrict__ qi
         StageParams s = sp;
         if constexpr (NZ) {
             const long fsz = 6L * NF * R.nfb;
-            const double* base = noise + R.noff;
+            const float* base = noise + R.noff;
             s.noise[0] = base;
             s.noise[1] = base + fsz;
             s.noise[2] = base + 2 * fsz;
@@ -130,7 +130,7 @@ struct EnsNoise {
     int nface, npg, nreal;
     const long* pts;
     const EnsReal* reals;
-    double* out;
+    float* out;
 };
 
 static __global__ __launch_bounds__(256) void k_ens_noise(EnsNoise np) {
@@ -147,7 +147,7 @@ static __global__ __launch_bounds__(256) void k_ens_noise(EnsNoise np) {
     const double sq = sqrt(np.dVi / *np.dt);
     long id, count;
     unsigned int stream;
-    double* out = np.out + R.noff;
+    float* out = np.out + R.noff;
     double kap;
     if (u < 3 * nf) {
         const int d = (int)(u / nf);
@@ -167,7 +167,7 @@ static __global__ __launch_bounds__(256) void k_ens_noise(EnsNoise np) {
     noise::stress(np.seed, R.realization, *np.step, stream, (unsigned long long)id, np.eta_sd * sq * kap,
                   np.zeta_sd * sq * kap, S);
 #pragma unroll
-    for (int e = 0; e < 6; e++) out[(long)e * count + id] = S[e];
+    for (int e = 0; e < 6; e++) out[(long)e * count + id] = (float)S[e];  // single precision, as k_noise
 }
 
 }  // namespace ens

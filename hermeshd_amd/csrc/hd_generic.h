@@ -274,7 +274,7 @@ struct StageParams {
     unsigned long long* bad;
     int limit_out;        // fused kernels: limit the output (it is the next stage's input)
     LimParams lp;
-    const double* noise[4];  // stochastic stress of the step: x, y, z face families, volume points
+    const float* noise[4];   // stochastic stress of the step (single precision): x, y, z face families, volume points
     long nfb;                // face lattice size (nx+1)(ny+1)(nz+1)
     const double* inflow;    // x-low inflow exterior state (ny,nz,nface,nQ), Fortran order
 };

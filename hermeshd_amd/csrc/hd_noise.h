@@ -86,8 +86,8 @@ struct NoiseParams {
     const double* dt;                // device time step
     double eta_sd, zeta_sd, dVi, kap_f, kap_v;
     long nfb, N;                     // face lattice size, cells
-    double* face[3];                 // [6][nface][nfb] per face family
-    double* vol;                     // [6][npg][N]
+    float* face[3];                  // [6][nface][nfb] per face family
+    float* vol;                      // [6][npg][N]
 };
 
 static __global__ __launch_bounds__(256) void k_noise(NoiseParams np) {
@@ -98,7 +98,7 @@ static __global__ __launch_bounds__(256) void k_noise(NoiseParams np) {
     const uint64_t step = *np.step;
     long id, count;
     uint32_t stream;
-    double* out;
+    float* out;
     double kap;
     if (t < 3 * nf) {
         const int d = (int)(t / nf);
@@ -116,8 +116,10 @@ static __global__ __launch_bounds__(256) void k_noise(NoiseParams np) {
     }
     double S[6];
     stress(np.seed, np.realization, step, stream, (uint64_t)id, np.eta_sd * sq * kap, np.zeta_sd * sq * kap, S);
+    // stored in single precision, the reference's precision for its
+    // Gaussian draws (vsRngGaussian, src/random.f90:61,87)
 #pragma unroll
-    for (int e = 0; e < 6; e++) out[(long)e * count + id] = S[e];
+    for (int e = 0; e < 6; e++) out[(long)e * count + id] = (float)S[e];
 }
 
 static __global__ void k_set_step(unsigned long long* step, unsigned long long value, int advance) {

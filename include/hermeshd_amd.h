@@ -101,8 +101,10 @@ int hd_min_dt(hd_ctx *ctx, const double *q, double *dt_dev, void *stream);
    context's own stage array is used).  The SSP combinations are fused into
    the stage kernels.  On return q_io holds the new state; q1 and q2 hold
    unspecified values.  With noise on, one stochastic stress field is drawn
-   for the step (key seed, realization, step index; amplitude from *dt_dev),
-   used by every stage, and the step index then advances by one. */
+   for the step (key seed, realization, step index; amplitude from *dt_dev;
+   computed in FP64, stored in FP32, the reference's precision for its
+   Gaussian draws), used by every stage, and the step index then advances by
+   one. */
 int hd_update(hd_ctx *ctx, double *q_io, double *q1, double *q2, const double *dt_dev, void *stream);
 
 /* dt = min_dt(q_io); update; t += dt.  t_dev and dt_dev are device doubles.

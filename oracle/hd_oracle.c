@@ -934,6 +934,9 @@ static void noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double 
     S[3] = amp_eta * g[3];
     S[4] = amp_eta * g[4];
     S[5] = amp_eta * g[5];
+    /* the field is stored in single precision, the reference's precision
+       for its Gaussian draws (vsRngGaussian, src/random.f90:61,87) */
+    for (int e = 0; e < 6; e++) S[e] = (double)(float)S[e];
 }
 
 int orc_noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta, double S[6]) {
