@@ -334,8 +334,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": int(traffic) if traffic is not None else None,
-                "kernel": "k_stage (fused RK stage: volume, faces, sources, update, SSP combination, "
-                          "next stage's limiter)",
+                "kernel": "k_stage_xs (fused RK stage, x faces shared along each periodic row: volume, "
+                          "faces, sources, update, SSP combination, next stage's limiter)",
                 "algorithmic_bytes_per_launch": int(bytes_stage), "mean_launch_us": round(mean_stage_ms * 1e3, 2),
                 "copy_probe_GBps": round(copy_gbps, 1)}
     if traffic_detail:
