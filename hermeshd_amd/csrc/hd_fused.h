@@ -222,7 +222,17 @@ __device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&
                                             double dxi, double dyi, double dzi, const StageParams& sp, long c,
                                             long N) {
 #pragma unroll
-    for (int i = 0; i < QP; i++)
+    for (int i = 0; i < QP; i++) {
+        // the stress of the plane's QP*QP points, loaded together so that
+        // the points do not each wait for memory
+        float Sv[QP * QP][6];
+        if constexpr (NZ) {
+            constexpr long npg = QP * QP * QP;
+#pragma unroll
+            for (int jk = 0; jk < QP * QP; jk++)
+#pragma unroll
+                for (int e = 0; e < 6; e++) Sv[jk][e] = sp.noise[3][((long)e * npg + i * QP * QP + jk) * N + c];
+        }
 #pragma unroll
         for (int j = 0; j < QP; j++)
 #pragma unroll
@@ -244,10 +254,8 @@ __device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&
                 const double* Sp = nullptr;
                 if constexpr (NZ) {
                     // volume point (i,j,k) of the cell: stream id p*N + c
-                    const int p = (i * QP + j) * QP + k;
-                    constexpr long npg = QP * QP * QP;
 #pragma unroll
-                    for (int e = 0; e < 6; e++) S[e] = sp.noise[3][((long)e * npg + p) * N + c];
+                    for (int e = 0; e < 6; e++) S[e] = Sv[j * QP + k][e];
                     Sp = S;
                 }
                 pr::flux<NQ, 0>(ph, Qp, st, Fx, Sp);
@@ -268,6 +276,7 @@ __device__ __forceinline__ void volume_term(const double (&m)[NQ][NB], double (&
                 }
                 point_barrier();
             }
+    }
 }
 
 // ------------------------------------------------------------- faces -------
@@ -277,11 +286,15 @@ struct CellPos {
 };
 
 template <int NB, int NQ>
-__device__ __forceinline__ void load_modes(const double* __restrict__ q, const Grid& g, long c, double (&m)[NQ][NB]) {
+__device__ __forceinline__ void load_modes(const double* __restrict__ q, long N, long c, double (&m)[NQ][NB]) {
 #pragma unroll
     for (int r = 0; r < NB; r++)
 #pragma unroll
-        for (int f = 0; f < NQ; f++) m[f][r] = q[((long)r * NQ + f) * g.N + c];
+        for (int f = 0; f < NQ; f++) m[f][r] = q[((long)r * NQ + f) * N + c];
+}
+template <int NB, int NQ>
+__device__ __forceinline__ void load_modes(const double* __restrict__ q, const Grid& g, long c, double (&m)[NQ][NB]) {
+    load_modes<NB, NQ>(q, g.N, c, m);
 }
 
 // Exterior state of an x-low inflow face point: inflow holds the reference's
@@ -348,6 +361,14 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
         const int fi = DIR == 0 ? af : cp.i, fj = DIR == 1 ? af : cp.j, fk = DIR == 2 ? af : cp.k;
         fid = (long)fi + (long)(g.nx + 1) * ((long)fj + (long)(g.ny + 1) * fk);
     }
+    // the stress of all the face's points, loaded before the point loop
+    float Sf[NZ ? NF : 1][6];
+    if constexpr (NZ) {
+#pragma unroll
+        for (int p = 0; p < NF; p++)
+#pragma unroll
+            for (int e = 0; e < 6; e++) Sf[p][e] = sp.noise[DIR][((long)e * NF + p) * sp.nfb + fid];
+    }
 #pragma unroll 1
     for (int p = 0; p < NF; p++) {
         double own[NQ], oth[NQ];
@@ -379,8 +400,15 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
         double S[6];
         const double* Sp = nullptr;
         if constexpr (NZ) {
+            // p is a loop counter (not unrolled): select rather than index,
+            // which keeps Sf in registers
 #pragma unroll
-            for (int e = 0; e < 6; e++) S[e] = sp.noise[DIR][((long)e * NF + p) * sp.nfb + fid];
+            for (int e = 0; e < 6; e++) S[e] = Sf[0][e];
+#pragma unroll
+            for (int pp = 1; pp < NF; pp++)
+                if (pp == p)
+#pragma unroll
+                    for (int e = 0; e < 6; e++) S[e] = Sf[pp][e];
             Sp = S;
         }
         double FL[NQ], FR[NQ], out[NQ];
@@ -560,6 +588,33 @@ __global__ __launch_bounds__(256, (NB == 4 ? 4 : 2)) void k_limit(double* __rest
 // ------------------------------------------------------------ kernel -------
 __device__ __forceinline__ void fence() { asm volatile("" ::: "memory"); }
 
+// advance_time_level and the SSP combination of one cell (src/integrator.f90:
+// 62-86, 210-232): rhs becomes m + dt*rhs (the stress fields scaled by faci
+// under ivis 1), combined with the step-start state qn by COMB (0: none,
+// 1: RK2 (qn+v)/2, 2: RK3 3qn/4 + v/4, 3: RK3 qn/3 + 2v/3).  Returns whether a
+// mean value came out NaN.
+template <int NB, int NQ, int COMB>
+__device__ __forceinline__ bool advance(const double (&m)[NQ][NB], double (&rhs)[NQ][NB],
+                                        const double* __restrict__ qn, long N, long c, double dt, double faci,
+                                        bool relax) {
+    double q0[COMB ? NQ : 1][COMB ? NB : 1];
+    if constexpr (COMB != 0) load_modes<NB, NQ>(qn, N, c, q0);
+    bool bad = false;
+#pragma unroll
+    for (int f = 0; f < NQ; f++)
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            double v = m[f][r] + dt * rhs[f][r];
+            if (f >= PXX && relax) v = faci * v;
+            if (r == 0 && v != v) bad = true;
+            if constexpr (COMB == 1) v = 0.5 * (q0[f][r] + v);
+            if constexpr (COMB == 2) v = 0.75 * q0[f][r] + 0.25 * v;
+            if constexpr (COMB == 3) v = (1. / 3.) * q0[f][r] + (2. / 3.) * v;
+            rhs[f][r] = v;  // reuse the accumulator as the output cell
+        }
+    return bad;
+}
+
 // One cell of one stage; `bad_id` is the index reported when mode 1 of the
 // output is NaN.
 // XS: the workgroup is one periodic row of cells (k_stage_xs); each x face is
@@ -628,19 +683,15 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const
             for (int r = 0; r < NB; r++) qout[((long)r * NQ + f) * g.N + c] = rhs[f][r];
         return;
     }
-#pragma unroll
-    for (int f = 0; f < NQ; f++)
-#pragma unroll
-        for (int r = 0; r < NB; r++) {
-            const long o = ((long)r * NQ + f) * g.N + c;
-            double v = m[f][r] + dt * rhs[f][r];
-            if (sp.ivis == 1 && f >= PXX) v = faci * v;
-            if (r == 0 && v != v) bad = true;
-            if (sp.combine == 1) v = 0.5 * (qn[o] + v);
-            else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
-            else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
-            rhs[f][r] = v;  // reuse the accumulator as the output cell
-        }
+    // one branch per combination, outside the element loop, so that the
+    // step-start values are loaded together rather than one per branch
+    const bool relax = NQ >= 11 && sp.ivis == 1;
+    switch (sp.combine) {
+        case 1: bad = advance<NB, NQ, 1>(m, rhs, qn, g.N, c, dt, faci, relax); break;
+        case 2: bad = advance<NB, NQ, 2>(m, rhs, qn, g.N, c, dt, faci, relax); break;
+        case 3: bad = advance<NB, NQ, 3>(m, rhs, qn, g.N, c, dt, faci, relax); break;
+        default: bad = advance<NB, NQ, 0>(m, rhs, qn, g.N, c, dt, faci, relax); break;
+    }
     // When this output is the next stage's input, apply that stage's limiter
     // here: the reference limits the stage input in place before anything
     // reads it (src/integrator.f90:106-113), so the stored values are the same.
