@@ -134,7 +134,7 @@ int ens_noise(hd_ens* e, const double* dt_dev, hipStream_t s) {
     np.reals = e->dreal;
     np.out = e->noise;
     const long total = e->hpts[e->nreal];
-    hipLaunchKernelGGL(ens::k_ens_noise, dim3(blocks_for(total, 256)), dim3(256), 0, s, np);
+    hipLaunchKernelGGL(ens::k_ens_noise, dim3(blocks_for(total, ens::NOISE_BLOCK)), dim3(ens::NOISE_BLOCK), 0, s, np);
     HIPCHK(hipGetLastError());
     return HD_OK;
 }
@@ -235,7 +235,8 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
         qoff += g.N * e->nQ * e->nb;
         cell0 += g.N;
         if (base->noise) noff += 3L * 6 * e->nface * R.nfb + 6L * e->npg * g.N;
-        e->hpts[r + 1] = e->hpts[r] + (base->noise ? 3L * e->nface * R.nfb + (long)e->npg * g.N : 0);
+        const long npts = base->noise ? 3L * e->nface * R.nfb + (long)e->npg * g.N : 0;
+        e->hpts[r + 1] = e->hpts[r] + (npts + ens::NOISE_BLOCK - 1) / ens::NOISE_BLOCK * ens::NOISE_BLOCK;
     }
     e->state_doubles = qoff;
     e->cells = cell0;

@@ -121,7 +121,9 @@ __global__ __launch_bounds__(256) void k_ens_limit(double* __restrict__ q, const
 
 // Noise of every realization for the current step: flat over all points;
 // `pts` holds the prefix sums of the per-realization point counts
-// (3*nface*nfb + npg*N), nreal+1 entries.
+// (3*nface*nfb + npg*N), each rounded up to a whole number of 256-thread
+// blocks so that a block lies in one realization (nreal+1 entries).
+constexpr long NOISE_BLOCK = 256;
 struct EnsNoise {
     unsigned long long seed;
     const unsigned long long* step;
@@ -133,24 +135,27 @@ struct EnsNoise {
     float* out;
 };
 
-static __global__ __launch_bounds__(256) void k_ens_noise(EnsNoise np) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= np.pts[np.nreal]) return;
-    int lo = 0, hi = np.nreal;  // pts[lo] <= t < pts[hi]
+static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
+    // the block's realization: a search on the block's first point, uniform
+    // over the block
+    const long t0 = (long)blockIdx.x * NOISE_BLOCK;
+    if (t0 >= np.pts[np.nreal]) return;
+    int lo = 0, hi = np.nreal;  // pts[lo] <= t0 < pts[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (np.pts[mid] <= t) lo = mid; else hi = mid;
+        if (np.pts[mid] <= t0) lo = mid; else hi = mid;
     }
     const EnsReal& R = np.reals[lo];
-    const long u = t - np.pts[lo];
+    const long u = t0 - np.pts[lo] + threadIdx.x;
     const long nf = (long)np.nface * R.nfb, nv = (long)np.npg * R.g.N;
+    if (u >= 3 * nf + nv) return;  // the padding of the last block
     const double sq = sqrt(np.dVi / *np.dt);
     long id, count;
     unsigned int stream;
     float* out = np.out + R.noff;
     double kap;
     if (u < 3 * nf) {
-        const int d = (int)(u / nf);
+        const int d = (u >= nf) + (u >= 2 * nf);
         id = u - (long)d * nf;
         count = nf;
         stream = (unsigned int)(d + 1);

@@ -101,7 +101,7 @@ static __global__ __launch_bounds__(256) void k_noise(NoiseParams np) {
     float* out;
     double kap;
     if (t < 3 * nf) {
-        const int d = (int)(t / nf);
+        const int d = (t >= nf) + (t >= 2 * nf);
         id = t - (long)d * nf;
         count = nf;
         stream = (uint32_t)(d + 1);
