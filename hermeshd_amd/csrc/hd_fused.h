@@ -315,10 +315,29 @@ __device__ __forceinline__ void inflow_ghost(const double* __restrict__ inflow, 
 // One face of the cell: DIR 0/1/2, SIDE 0 (minus) / 1 (plus).
 // STORE: also write each point's scaled flux to xs[(p*NQ + f)*nx + i] for the
 // x-neighbour's lift (the row-shared x faces of k_stage_xs).
-template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE, bool STORE = false>
+// Modes of the neighbour across the DIR face on side SIDE (an interior or
+// periodic face).
+template <int NB, int NQ, int DIR, int SIDE>
+__device__ __forceinline__ void load_neighbor(const double* __restrict__ q, const Grid& g, const CellPos& cp,
+                                              double (&nm)[NQ][NB]) {
+    const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int a = DIR == 0 ? cp.i : DIR == 1 ? cp.j : cp.k;
+    int an = SIDE ? a + 1 : a - 1;
+    if (an < 0) an += n_ax;
+    if (an >= n_ax) an -= n_ax;
+    const long cn = DIR == 0 ? (cp.c - cp.i + an)
+                  : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
+                             : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
+    load_modes<NB, NQ>(q, g, cn, nm);
+}
+
+// PRE: the neighbour's modes were loaded earlier (pre), so that their
+// latency overlapped other work.
+template <int NB, int QP, int NQ, int RM, bool NZ, int DIR, int SIDE, bool STORE = false, bool PRE = false>
 __device__ __forceinline__ void face_term(const double* __restrict__ q, const Grid& g, const CellPos& cp,
                                           const double (&m)[NQ][NB], double (&rhs)[NQ][NB], const Phys& ph,
-                                          double dxi_d, const StageParams& sp, double* xs = nullptr) {
+                                          double dxi_d, const StageParams& sp, double* xs = nullptr,
+                                          const double (*pre)[NB] = nullptr) {
     constexpr int NF = QP * QP;
     const auto& T = face_tab<NB>();
     const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
@@ -331,14 +350,15 @@ __device__ __forceinline__ void face_term(const double* __restrict__ q, const Gr
     double ko[NQ][NK], kn[NQ][NK];
     face_coeffs<NB, NQ, DIR, SIDE, NK>(m, ko);
     if (!ext) {
-        int an = SIDE ? a + 1 : a - 1;
-        if (an < 0) an += n_ax;
-        if (an >= n_ax) an -= n_ax;
-        const long cn = DIR == 0 ? (cp.c - cp.i + an)
-                      : DIR == 1 ? (cp.c + (long)(an - cp.j) * g.nx)
-                                 : (cp.c + (long)(an - cp.k) * g.nx * g.ny);
         double nm[NQ][NB];
-        load_modes<NB, NQ>(q, g, cn, nm);
+        if constexpr (PRE) {
+#pragma unroll
+            for (int f = 0; f < NQ; f++)
+#pragma unroll
+                for (int r = 0; r < NB; r++) nm[f][r] = pre[f][r];
+        } else {
+            load_neighbor<NB, NQ, DIR, SIDE>(q, g, cp, nm);
+        }
         face_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nm, kn);
     }
     // outflow boundaries decide on the whole face (src/boundary.f90:213-338)
@@ -633,15 +653,23 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const
 #pragma unroll
         for (int r = 0; r < NB; r++) rhs[f][r] = 0.;
     // The fences keep each section's loads inside it, so the compiler does not
-    // hoist all six neighbours' modes at once (which would spill).
+    // hoist all six neighbours' modes at once (which would spill).  The first
+    // face's neighbour is loaded ahead of the volume term, whose arithmetic
+    // covers that load's latency.
+    double nm0[NQ][NB];
+    if constexpr (XS) {
+        load_neighbor<NB, NQ, 0, 1>(qin, g, cp, nm0);
+    } else {
+        if (!(cp.i == 0 && g.bc[0] != BC_PERIODIC)) load_neighbor<NB, NQ, 0, 0>(qin, g, cp, nm0);
+    }
     volume_term<NB, QP, NQ, NZ>(m, rhs, ph, g.dxi, g.dyi, g.dzi, sp, c, g.N);
     fence();
     if constexpr (XS) {
-        face_term<NB, QP, NQ, RM, NZ, 0, 1, true>(qin, g, cp, m, rhs, ph, g.dxi, sp, xs);
+        face_term<NB, QP, NQ, RM, NZ, 0, 1, true, true>(qin, g, cp, m, rhs, ph, g.dxi, sp, xs, nm0);
         __syncthreads();
         x_lift_shared<NB, QP, NQ>(xs, g, cp, rhs);
     } else {
-        face_term<NB, QP, NQ, RM, NZ, 0, 0>(qin, g, cp, m, rhs, ph, g.dxi, sp);
+        face_term<NB, QP, NQ, RM, NZ, 0, 0, false, true>(qin, g, cp, m, rhs, ph, g.dxi, sp, nullptr, nm0);
         fence();
         face_term<NB, QP, NQ, RM, NZ, 0, 1>(qin, g, cp, m, rhs, ph, g.dxi, sp);
     }
