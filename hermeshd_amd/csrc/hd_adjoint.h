@@ -592,15 +592,28 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
                     }
             }
             if (ap.ivis == 1 && ap.coll != 0. && ap.out) {
-                // v = faci*(m + dt*R): dv/dcoll = -dt*faci*v, b*v = out - a*Q^n
+                // v = faci*(m + dt*R): dv/dcoll = -dt*faci*v, b*v = out - a*Q^n;
+                // the branch on a stays outside the element loop so that the
+                // loads are issued together
+                if (ap.a != 0.) {
 #pragma unroll
-                for (int f = PXX; f <= PYZ; f++)
+                    for (int f = PXX; f <= PYZ; f++)
 #pragma unroll
-                    for (int r = 0; r < NB; r++) {
-                        const long o = ((long)r * NQ + f) * g.N + c;
-                        const double bv = ap.out[o] - (ap.a != 0. ? ap.a * ap.qn[o] : 0.);
-                        dcoll += -dt * faci * lam[o] * bv;
-                    }
+                        for (int r = 0; r < NB; r++) {
+                            const long o = ((long)r * NQ + f) * g.N + c;
+                            const double bv = ap.out[o] - ap.a * ap.qn[o];
+                            dcoll += -dt * faci * lam[o] * bv;
+                        }
+                } else {
+#pragma unroll
+                    for (int f = PXX; f <= PYZ; f++)
+#pragma unroll
+                        for (int r = 0; r < NB; r++) {
+                            const long o = ((long)r * NQ + f) * g.N + c;
+                            const double bv = ap.out[o] - 0.;
+                            dcoll += -dt * faci * lam[o] * bv;
+                        }
+                }
             }
         }
         if (ap.bf[0] != 0. || ap.bf[1] != 0. || ap.bf[2] != 0.) {
@@ -648,13 +661,19 @@ __device__ __forceinline__ void cell_vjp_finish(double* __restrict__ lam_in, con
 #pragma unroll
     for (int f = 0; f < NQ; f++)
 #pragma unroll
-        for (int r = 0; r < NB; r++) {
-            const long o = ((long)r * NQ + f) * g.N + c;
-            double v = lo[f][r] + dt * rt[f][r];
-            if (ap.acc[0]) v += ap.w[0] * ap.acc[0][o];
-            if (ap.acc[1]) v += ap.w[1] * ap.acc[1][o];
-            rt[f][r] = v;
+        for (int r = 0; r < NB; r++) rt[f][r] = lo[f][r] + dt * rt[f][r];
+    // each accumulated term under one branch, its values loaded together
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        if (ap.acc[t]) {
+            double av[NQ][NB];
+            fz::load_modes<NB, NQ>(ap.acc[t], g, c, av);
+#pragma unroll
+            for (int f = 0; f < NQ; f++)
+#pragma unroll
+                for (int r = 0; r < NB; r++) rt[f][r] += ap.w[t] * av[f][r];
         }
+    }
     if (ap.q_raw) {
         double raw[NQ][NB];
         fz::load_modes<NB, NQ>(ap.q_raw, g, c, raw);
