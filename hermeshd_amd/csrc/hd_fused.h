@@ -589,7 +589,7 @@ __device__ __forceinline__ bool limit_cell(double (&m)[MQ][NB], const LimParams&
 }
 
 template <int NB, int QP>
-__global__ __launch_bounds__(256, (NB == 4 ? 4 : 2)) void k_limit(double* __restrict__ q, Grid g, LimParams lp) {
+__global__ __launch_bounds__(256, (NB == 4 ? 4 : (NB == 8 ? 2 : 1))) void k_limit(double* __restrict__ q, Grid g, LimParams lp) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     double m[5][NB];
