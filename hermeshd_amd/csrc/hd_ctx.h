@@ -93,6 +93,10 @@ struct hd_ens {
     hd::ens::Launch launch{};
     // captured step (hd_ens_run)
     hipStream_t cap_stream = nullptr;
+    // the next step's noise is drawn on side_stream while the captured
+    // step's stages run (fork/join events inside the capture)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     hipGraphExec_t gexec = nullptr;
     const void* gkey[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int gchoice = -1;

@@ -116,11 +116,14 @@ StageParams ens_params(const hd_ens* e, const double* dt_dev, int combine, int m
     return sp;
 }
 
-int ens_noise(hd_ens* e, const double* dt_dev, hipStream_t s) {
+// Draws the field of step (*step_ctr + ahead) into buffer (that step & 1).
+int ens_noise(hd_ens* e, const double* dt_dev, hipStream_t s, int ahead = 0) {
     if (!e->base.noise) return HD_OK;
     ens::EnsNoise np;
     np.seed = e->base.seed;
     np.step = e->step_ctr;
+    np.ahead = ahead;
+    np.nfloats = e->noise_floats;
     np.dt = dt_dev;
     np.eta_sd = e->base.eta_sd;
     np.zeta_sd = e->base.zeta_sd;
@@ -150,20 +153,39 @@ int ens_stage(hd_ens* e, const Ops& o, double* qin, const double* qn, double* qo
 }
 
 // One step, stream-ordered: noise, stages with fused SSP combinations, step index.
-int ens_step(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt_dev, hipStream_t s) {
+// pipelined: the step's field is already drawn (buffer step & 1); the next
+// step's field is drawn on side_stream, forked from s and joined before the
+// step index advances, so that the draw overlaps the stages.
+int ens_step(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt_dev, hipStream_t s,
+             bool pipelined = false) {
     Ops o;
     if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     int rc;
-    if ((rc = ens_noise(e, dt_dev, s))) return rc;
+    const bool fork = pipelined && e->base.noise;
+    if (!fork && (rc = ens_noise(e, dt_dev, s))) return rc;
+    // The draw depends only on what precedes the step (fork_ev), but is
+    // issued after the first stage, so that the stage's workgroups are placed
+    // first and the draw fills the register space they leave.
+    if (fork) HIPCHK(hipEventRecord(e->fork_ev, s));
+    auto fork_noise = [&]() -> int {
+        HIPCHK(hipStreamWaitEvent(e->side_stream, e->fork_ev, 0));
+        int r = ens_noise(e, dt_dev, e->side_stream, 1);
+        if (r) return r;
+        HIPCHK(hipEventRecord(e->join_ev, e->side_stream));
+        return HD_OK;
+    };
     if (e->base.integrator == 3) {
         double* q2b = q2 ? q2 : e->stage_buf;
         if ((rc = ens_stage(e, o, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if (fork && (rc = fork_noise())) return rc;
         if ((rc = ens_stage(e, o, q1, q_io, q2b, dt_dev, 2, s, true, true))) return rc;
         if ((rc = ens_stage(e, o, q2b, q_io, q_io, dt_dev, 3, s, true, false))) return rc;
     } else {
         if ((rc = ens_stage(e, o, q_io, q_io, q1, dt_dev, 0, s, false, true))) return rc;
+        if (fork && (rc = fork_noise())) return rc;
         if ((rc = ens_stage(e, o, q1, q_io, q_io, dt_dev, 1, s, true, false))) return rc;
     }
+    if (fork) HIPCHK(hipStreamWaitEvent(s, e->join_ev, 0));
     if (e->base.noise) {
         hipLaunchKernelGGL(noise::k_set_step, dim3(1), dim3(64), 0, s, e->step_ctr, 0ull, 1);
         HIPCHK(hipGetLastError());
@@ -248,7 +270,7 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
         (rc = ens_alloc(e, (void**)&e->step_ctr, sizeof(unsigned long long))) ||
         (rc = ens_alloc(e, (void**)&e->err, sizeof(int))) ||
         (rc = ens_alloc(e, (void**)&e->bad, sizeof(unsigned long long))) ||
-        (noff > 0 && (rc = ens_alloc(e, (void**)&e->noise, noff * sizeof(float)))) ||
+        (noff > 0 && (rc = ens_alloc(e, (void**)&e->noise, 2 * noff * sizeof(float)))) ||
         (base->integrator == 3 && (rc = ens_alloc(e, (void**)&e->stage_buf, qoff * sizeof(double))))) {
         std::string msg = g_last_error;
         hd_ens_destroy(e);
@@ -260,7 +282,10 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
         hipMemset(e->step_ctr, 0, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(e->err, 0, sizeof(int)) != hipSuccess ||
         hipMemset(e->bad, 0xff, sizeof(unsigned long long)) != hipSuccess ||
-        hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join_ev, hipEventDisableTiming) != hipSuccess) {
         hd_ens_destroy(e);
         return fail(HD_ERR_HIP, "initialising ensemble buffers failed");
     }
@@ -277,6 +302,9 @@ int hd_ens_destroy(hd_ens* e) {
     if (!e) return HD_OK;
     drop_graph(e);
     if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
+    if (e->side_stream) (void)hipStreamDestroy(e->side_stream);
+    if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
+    if (e->join_ev) (void)hipEventDestroy(e->join_ev);
     void* ptrs[] = {e->dreal, e->dpts, e->dwork, e->noise, e->one_dt, e->stage_buf, e->step_ctr, e->err, e->bad};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -332,7 +360,7 @@ int hd_ens_run(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt
         // capture one step on the ensemble's own stream; the graph holds
         // kernel nodes only
         HIPCHK(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
-        int rc = ens_step(e, q_io, q1, q2, dt_dev, e->cap_stream);
+        int rc = ens_step(e, q_io, q1, q2, dt_dev, e->cap_stream, true);
         hipGraph_t graph = nullptr;
         hipError_t ce = hipStreamEndCapture(e->cap_stream, &graph);
         if (rc) {
@@ -348,6 +376,12 @@ int hd_ens_run(hd_ens* e, double* q_io, double* q1, double* q2, const double* dt
         }
         std::memcpy(e->gkey, key, sizeof(key));
         e->gchoice = e->choice;
+    }
+    // each replay draws the field of the step after it: the first step's
+    // field is drawn here
+    if (nsteps > 0) {
+        int rc = ens_noise(e, dt_dev, (hipStream_t)stream);
+        if (rc) return rc;
     }
     for (int64_t n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(e->gexec, (hipStream_t)stream));
     return HD_OK;

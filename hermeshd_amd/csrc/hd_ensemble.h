@@ -73,7 +73,8 @@ template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_ens_stage(const double* __restrict__ qin, const double* __restrict__ qn,
                                                    double* __restrict__ qout, const EnsReal* __restrict__ reals,
                                                    const int2* __restrict__ work, Launch L, Phys ph,
-                                                   StageParams sp, const float* __restrict__ noise) {
+                                                   StageParams sp, const float* __restrict__ noise,
+                                                   const unsigned long long* __restrict__ step, long nfloats) {
     constexpr int NF = QP * QP;
     for_each_item(L, [&](int w) {
         const int2 wk = work[w];
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(256) void k_ens_stage(const double* __restrict__ qi
         StageParams s = sp;
         if constexpr (NZ) {
             const long fsz = 6L * NF * R.nfb;
-            const float* base = noise + R.noff;
+            // the field of the current step: buffer (step & 1)
+            const float* base = noise + ((*step & 1ull) ? nfloats : 0L) + R.noff;
             s.noise[0] = base;
             s.noise[1] = base + fsz;
             s.noise[2] = base + 2 * fsz;
@@ -127,6 +129,8 @@ constexpr long NOISE_BLOCK = 256;
 struct EnsNoise {
     unsigned long long seed;
     const unsigned long long* step;
+    int ahead;       // draws step *step + ahead
+    long nfloats;    // floats of one step's field; step s lives in buffer s & 1
     const double* dt;
     double eta_sd, zeta_sd, dVi, kap_f, kap_v;
     int nface, npg, nreal;
@@ -150,9 +154,10 @@ static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
     const long nf = (long)np.nface * R.nfb, nv = (long)np.npg * R.g.N;
     if (u >= 3 * nf + nv) return;  // the padding of the last block
     const double sq = sqrt(np.dVi / *np.dt);
+    const unsigned long long st = *np.step + (unsigned long long)np.ahead;
     long id, count;
     unsigned int stream;
-    float* out = np.out + R.noff;
+    float* out = np.out + ((st & 1ull) ? np.nfloats : 0L) + R.noff;
     double kap;
     if (u < 3 * nf) {
         const int d = (u >= nf) + (u >= 2 * nf);
@@ -169,7 +174,7 @@ static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
         kap = np.kap_v;
     }
     double S[6];
-    noise::stress(np.seed, R.realization, *np.step, stream, (unsigned long long)id, np.eta_sd * sq * kap,
+    noise::stress(np.seed, R.realization, st, stream, (unsigned long long)id, np.eta_sd * sq * kap,
                   np.zeta_sd * sq * kap, S);
 #pragma unroll
     for (int e = 0; e < 6; e++) out[(long)e * count + id] = (float)S[e];  // single precision, as k_noise

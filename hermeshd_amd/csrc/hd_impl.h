@@ -59,12 +59,13 @@ struct Impl {
         if constexpr (NQ >= 11) {
             if (e->base.noise) {
                 hipLaunchKernelGGL((ens::k_ens_stage<NB, QP, NQ, RM, true>), dim3(e->grid), dim3(e->bs), 0, s, qin,
-                                   qn, qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise);
+                                   qn, qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise, e->step_ctr,
+                                   e->noise_floats);
                 return;
             }
         }
         hipLaunchKernelGGL((ens::k_ens_stage<NB, QP, NQ, RM, false>), dim3(e->grid), dim3(e->bs), 0, s, qin, qn,
-                           qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise);
+                           qout, e->dreal, e->dwork, e->launch, e->ph, sp, e->noise, e->step_ctr, e->noise_floats);
     }
     static void stage_vjp(hd_ctx* c, const double* q, const double* lam, double* lam_in, const adj::AdjParams& ap,
                           hipStream_t s) {
