@@ -242,7 +242,10 @@ int hd_ens_update(hd_ens *ens, double *q_io, double *q1, double *q2, const doubl
 
 /* nsteps steps of hd_ens_update, replayed from a HIP graph of one step that
    is captured on the first call (and again when an argument or the launch
-   shape changes).  *dt_dev may change between calls. */
+   shape changes).  *dt_dev may change between calls.  With noise on, the call
+   first draws the current step's field; each replayed step then draws the
+   next step's field on the ensemble's own side stream while its stages run
+   (the field is double-buffered), so results equal hd_ens_update's. */
 int hd_ens_run(hd_ens *ens, double *q_io, double *q1, double *q2, const double *dt_dev,
                int64_t nsteps, void *stream);
 
