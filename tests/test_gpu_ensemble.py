@@ -149,6 +149,25 @@ def test_ensemble_graph_replay_equals_eager(torch_mod):
     assert torch_mod.equal(qa, qb)
 
 
+def test_ensemble_graph_replay_new_dt(torch_mod):
+    """A replayed step draws the next step's noise ahead of time; a later call
+    with another dt redraws it, so replays still equal eager steps when dt
+    changes between calls."""
+    from hermeshd_amd.ensemble import Ensemble
+    base = CASES["noise_nb4_hllc_rk2"]
+    ens = Ensemble(base, DIMS)
+    q0 = ens.pack(states(base, DIMS, seed=43))
+    qa, qa1 = q0.clone(), ens.zeros()
+    for dt in (1e-3, 1e-3, 2e-3, 2e-3):
+        ens.update(qa, qa1, None, dt)
+    ens.set_step(0)
+    qb, qb1 = q0.clone(), ens.zeros()
+    ens.run(qb, qb1, None, 1e-3, 2)
+    ens.run(qb, qb1, None, 2e-3, 2)
+    ens.check()
+    assert torch_mod.equal(qa, qb)
+
+
 def test_ensemble_tuner(torch_mod):
     from hermeshd_amd.ensemble import Ensemble, launch_shapes
     base = CASES["noise_nb4_hllc_rk2"]
