@@ -137,7 +137,10 @@ int ens_noise(hd_ens* e, const double* dt_dev, hipStream_t s, int ahead = 0) {
     np.reals = e->dreal;
     np.out = e->noise;
     const long total = e->hpts[e->nreal];
-    hipLaunchKernelGGL(ens::k_ens_noise, dim3(blocks_for(total, ens::NOISE_BLOCK)), dim3(ens::NOISE_BLOCK), 0, s, np);
+    // ahead of time (beside the stages): one block per CU; otherwise one
+    // block per chunk
+    const unsigned grid = ahead ? (unsigned)std::max(1, e->num_cus) : blocks_for(total, ens::NOISE_BLOCK);
+    hipLaunchKernelGGL(ens::k_ens_noise, dim3(grid), dim3(ens::NOISE_BLOCK), 0, s, np);
     HIPCHK(hipGetLastError());
     return HD_OK;
 }

@@ -139,10 +139,11 @@ struct EnsNoise {
     float* out;
 };
 
-static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
-    // the block's realization: a search on the block's first point, uniform
-    // over the block
-    const long t0 = (long)blockIdx.x * NOISE_BLOCK;
+// One chunk of NOISE_BLOCK points (chunk b covers points b*NOISE_BLOCK ...).
+__device__ __forceinline__ void ens_noise_chunk(const EnsNoise& np, long b) {
+    // the chunk's realization: a search on its first point, uniform over the
+    // block
+    const long t0 = b * NOISE_BLOCK;
     if (t0 >= np.pts[np.nreal]) return;
     int lo = 0, hi = np.nreal;  // pts[lo] <= t0 < pts[hi]
     while (hi - lo > 1) {
@@ -178,6 +179,14 @@ static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
                   np.zeta_sd * sq * kap, S);
 #pragma unroll
     for (int e = 0; e < 6; e++) out[(long)e * count + id] = (float)S[e];  // single precision, as k_noise
+}
+
+// Blocks stride over the chunks: one chunk per block when the grid covers
+// them all, or a small grid (one block per CU) that stays beside a running
+// stage kernel and draws the next step's field in the stage's shadow.
+static __global__ __launch_bounds__(NOISE_BLOCK) void k_ens_noise(EnsNoise np) {
+    const long nchunk = np.pts[np.nreal] / NOISE_BLOCK;
+    for (long b = blockIdx.x; b < nchunk; b += gridDim.x) ens_noise_chunk(np, b);
 }
 
 }  // namespace ens
