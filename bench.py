@@ -213,14 +213,17 @@ def ensemble_leg(steps=20):
         sols.append(Solver(cfg))
         qs.append(torch.from_numpy(states[r]).cuda())
         q1s.append(torch.zeros_like(qs[-1]))
-    for s_, a, b in zip(sols, qs, q1s):
-        s_.update(a, b, None, 0.1)
+    # one device dt per context, filled once: each realization-step then costs
+    # what a batched step costs per realization (no per-call dt fill)
+    dts = [torch.full((), 0.1, dtype=torch.float64, device=qs[0].device) for _ in sols]
+    for s_, a, b, d in zip(sols, qs, q1s, dts):
+        s_.update(a, b, None, d)
     torch.cuda.synchronize()
     n1 = max(2, steps // 4)
     t0 = time.perf_counter()
     for _ in range(n1):
-        for s_, a, b in zip(sols, qs, q1s):
-            s_.update(a, b, None, 0.1)
+        for s_, a, b, d in zip(sols, qs, q1s, dts):
+            s_.update(a, b, None, d)
     torch.cuda.synchronize()
     t1 = (time.perf_counter() - t0) / n1
     dofs = ens.cells * base.nQ * base.nbasis
@@ -243,8 +246,8 @@ def main():
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how "
-              "the grid would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the grid would shard "
+              "by slabs with one face-trace exchange per stage); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from hermeshd_amd import ics

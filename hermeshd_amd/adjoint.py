@@ -48,10 +48,13 @@ class AdjointRunner:
 
     def _dt(self, s: Solver) -> torch.Tensor:
         """The time step as a device scalar (passed as a tensor, no fill per call)."""
+        # keyed on the value too: dt may be changed on the runner between runs
+        key = (s.device, float(self.dt))
         t = getattr(self, "_dt_dev", None)
-        if t is None or t.device != s.device:
+        if t is None or getattr(self, "_dt_key", None) != key:
             t = torch.full((), float(self.dt), dtype=torch.float64, device=s.device)
             self._dt_dev = t
+            self._dt_key = key
         return t
 
     def solver(self, coll: float) -> Solver:

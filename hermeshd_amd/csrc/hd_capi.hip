@@ -251,10 +251,20 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     const size_t nfx = (size_t)(g.nx + 1) * g.ny * g.nz, nfy = (size_t)g.nx * (g.ny + 1) * g.nz,
                  nfz = (size_t)g.nx * g.ny * (g.nz + 1);
     c->npart = 1024;
+    // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
+    // table-driven kernels only; HD_KERNELS=generic selects them for any configuration
+    const char* kern = std::getenv("HD_KERNELS");
+    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2;
+    if (c->generic && cfg->noise) {
+        hd_destroy(c);
+        return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
+    }
+    // face-flux arrays: only the table-driven kernels write them (the fused
+    // stage kernel keeps its face fluxes in registers and LDS)
     if ((r = alloc(c, (void**)&c->dtab, sizeof(Tables))) ||
-        (r = alloc(c, (void**)&c->fl[0], nf * nfx * sizeof(double))) ||
-        (r = alloc(c, (void**)&c->fl[1], nf * nfy * sizeof(double))) ||
-        (r = alloc(c, (void**)&c->fl[2], nf * nfz * sizeof(double))) ||
+        (c->generic && ((r = alloc(c, (void**)&c->fl[0], nf * nfx * sizeof(double))) ||
+                        (r = alloc(c, (void**)&c->fl[1], nf * nfy * sizeof(double))) ||
+                        (r = alloc(c, (void**)&c->fl[2], nf * nfz * sizeof(double))))) ||
         (r = alloc(c, (void**)&c->partial, c->npart * sizeof(double))) ||
         (r = alloc(c, (void**)&c->err, sizeof(int))) ||
         (r = alloc(c, (void**)&c->bad, sizeof(unsigned long long))) ||
@@ -304,14 +314,6 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         const bool off = xs && std::strcmp(xs, "0") == 0;
         if (!off && cfg->bc[0] == HD_BC_PERIODIC && cfg->nx % 64 == 0 && cfg->nx <= 256 && bytes <= 65536)
             c->xs_bytes = (unsigned)bytes;
-    }
-    const char* kern = std::getenv("HD_KERNELS");
-    // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
-    // table-driven kernels only
-    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2;
-    if (c->generic && cfg->noise) {
-        hd_destroy(c);
-        return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
     }
     *out = c;
     return HD_OK;

@@ -448,6 +448,13 @@ int hd_ens_tune(hd_ens* e, double* q, double* scratch, const double* dt_dev, int
     Ops o;
     if (!pick_ops_for(e->nb, e->q, e->nQ, e->base.riemann, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
     hipStream_t s = (hipStream_t)stream;
+    // a NaN flag raised by earlier stages and not yet read by hd_ens_check is
+    // kept: the tuning stages' own flags are discarded and this one restored
+    HIPCHK(hipStreamSynchronize(s));
+    int flag0 = 0;
+    unsigned long long bad0 = ~0ull;
+    HIPCHK(hipMemcpy(&flag0, e->err, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&bad0, e->bad, sizeof(bad0), hipMemcpyDeviceToHost));
     int rc = ens_noise(e, dt_dev, s);
     if (rc) return rc;
     hipEvent_t e0, e1;
@@ -477,10 +484,10 @@ int hd_ens_tune(hd_ens* e, double* q, double* scratch, const double* dt_dev, int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    // the tuning stages read the state only; clear any flag they raised
+    // the tuning stages read the state only: put back the flag they found
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemset(e->err, 0, sizeof(int)));
-    HIPCHK(hipMemset(e->bad, 0xff, sizeof(unsigned long long)));
+    HIPCHK(hipMemcpy(e->err, &flag0, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->bad, &bad0, sizeof(bad0), hipMemcpyHostToDevice));
     if ((rc = set_launch(e, best))) return rc;
     if (choice) *choice = best;
     return HD_OK;

@@ -107,7 +107,9 @@ class Hermeshd:
     @staticmethod
     def setup(q_io, t, dt, t1, t_start, dtout, nout, comm=None):
         """setup (src/hermeshd.f90:67-114): tables and context, initial
-        condition (when icid is configured), integrator and boundaries."""
+        condition (when icid is configured), integrator and boundaries.
+        comm (the reference's MPI communicator) is accepted for the call's
+        signature and not used: one device holds the whole grid."""
         cfg = _cfg()
         q_ic = inflow = mask = None
         if _input["icid"] is not None and not _input["iread"]:
@@ -153,7 +155,13 @@ class Hermeshd:
 
     @staticmethod
     def step(q_io, q_1, q_2, t, dt):
-        """step (src/hermeshd.f90:53-62): dt = get_min_dt(q_io); update; t += dt."""
+        """step (src/hermeshd.f90:53-62): dt = get_min_dt(q_io); update; t += dt.
+
+        Stage arrays: on return q_1 holds the first stage's output, limited
+        (for SSP-RK2 that is the reference's Q_r1 after its step); q_2 holds
+        unspecified values, because the SSP combination is fused into the
+        stage kernels and the uncombined last stage is never stored.  A numpy
+        q_1 receives the values from the device after the step."""
         s: Solver = _state["solver"]
         if s is None:
             raise RuntimeError("setup has not been called")
@@ -180,6 +188,10 @@ class Hermeshd:
             _set_scalar(t, float(s.t.item()))
         if back is not None:
             back(qd)
+        if isinstance(q_1, np.ndarray):
+            if q_1.shape != (cfg.nx, cfg.ny, cfg.nz, cfg.nQ, cfg.nbasis) or q_1.dtype != np.float64:
+                raise ValueError("numpy q_1 must be float64 with shape (nx,ny,nz,nQ,nbasis)")
+            q_1.T[...] = q1d.cpu().numpy()
 
     @staticmethod
     def generate_output(q_r, t, dt, t1, dtout, nout):

@@ -42,6 +42,40 @@ def build(force: bool = False) -> str:
 
 
 _lib = None
+_libs = {}
+
+
+def _configure(L):
+    dp = C.POINTER(C.c_double)
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [C.POINTER(OrcConfig)]
+    L.orc_destroy.argtypes = [C.c_void_p]
+    for name in ("orc_limiter",):
+        getattr(L, name).argtypes = [C.c_void_p, dp]
+    for name in ("orc_glflux", "orc_source", "orc_rhs"):
+        getattr(L, name).argtypes = [C.c_void_p, dp, dp]
+    L.orc_euler_step.argtypes = [C.c_void_p, dp, dp, C.c_double]
+    L.orc_min_dt.argtypes = [C.c_void_p, dp]
+    L.orc_min_dt.restype = C.c_double
+    L.orc_update.argtypes = [C.c_void_p, dp, dp, dp, C.c_double]
+    L.orc_step.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
+    L.orc_flux_points.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int]
+    L.orc_riemann_face.argtypes = [C.c_void_p, dp, dp, dp, C.c_int, C.c_int]
+    L.orc_cfcal.argtypes = [C.c_void_p, dp, C.c_int]
+    L.orc_cfcal.restype = C.c_double
+    L.orc_tables.argtypes = [C.c_void_p] + [dp] * 13
+    L.orc_set_step_index.argtypes = [C.c_void_p, C.c_uint64]
+    L.orc_set_noise_dt.argtypes = [C.c_void_p, C.c_double]
+    L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.orc_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, dp]
+    L.orc_noise_stress.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double, dp]
+    L.orc_set_inflow.argtypes = [C.c_void_p, dp]
+    L.orc_set_mask.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+    L.orc_apply_mask.argtypes = [C.c_void_p, dp]
+    L.orc_sample.argtypes = [C.c_void_p, dp, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_last_bad_cell.restype = C.c_int64
+    return L
 
 
 def lib():
@@ -49,38 +83,16 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        dp = C.POINTER(C.c_double)
-        L.orc_create.restype = C.c_void_p
-        L.orc_create.argtypes = [C.POINTER(OrcConfig)]
-        L.orc_destroy.argtypes = [C.c_void_p]
-        for name in ("orc_limiter",):
-            getattr(L, name).argtypes = [C.c_void_p, dp]
-        for name in ("orc_glflux", "orc_source", "orc_rhs"):
-            getattr(L, name).argtypes = [C.c_void_p, dp, dp]
-        L.orc_euler_step.argtypes = [C.c_void_p, dp, dp, C.c_double]
-        L.orc_min_dt.argtypes = [C.c_void_p, dp]
-        L.orc_min_dt.restype = C.c_double
-        L.orc_update.argtypes = [C.c_void_p, dp, dp, dp, C.c_double]
-        L.orc_step.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
-        L.orc_flux_points.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int]
-        L.orc_riemann_face.argtypes = [C.c_void_p, dp, dp, dp, C.c_int, C.c_int]
-        L.orc_cfcal.argtypes = [C.c_void_p, dp, C.c_int]
-        L.orc_cfcal.restype = C.c_double
-        L.orc_tables.argtypes = [C.c_void_p] + [dp] * 13
-        L.orc_set_step_index.argtypes = [C.c_void_p, C.c_uint64]
-        L.orc_set_noise_dt.argtypes = [C.c_void_p, C.c_double]
-        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
-        L.orc_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, dp]
-        L.orc_noise_stress.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double, dp]
-        L.orc_set_inflow.argtypes = [C.c_void_p, dp]
-        L.orc_set_mask.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
-        L.orc_apply_mask.argtypes = [C.c_void_p, dp]
-        L.orc_sample.argtypes = [C.c_void_p, dp, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
-        L.orc_last_error.restype = C.c_char_p
-        L.orc_last_bad_cell.restype = C.c_int64
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+def load_library(path: str):
+    """A separately built oracle library (the mutation tests build variants of
+    hd_oracle.c with one planted mistake each and check that the pins fail)."""
+    if path not in _libs:
+        _libs[path] = _configure(C.CDLL(path))
+    return _libs[path]
 
 
 def _p(a: np.ndarray):
@@ -96,7 +108,8 @@ class Oracle:
     """One oracle context (tables + scratch) for a configuration dict as
     produced by ``HDConfig.as_ints()``."""
 
-    def __init__(self, cfg: dict):
+    def __init__(self, cfg: dict, library=None):
+        self.L = library if library is not None else lib()
         self.cfg = dict(cfg)
         s = OrcConfig()
         for k, v in cfg.items():
@@ -105,80 +118,81 @@ class Oracle:
             else:
                 setattr(s, k, v)
         self._s = s
-        self.h = lib().orc_create(C.byref(s))
+        self.h = self.L.orc_create(C.byref(s))
         if not self.h:
-            raise OracleError(lib().orc_last_error().decode())
+            raise OracleError(self.L.orc_last_error().decode())
         self.shape = (cfg["nbasis"], cfg["nQ"], cfg["nz"], cfg["ny"], cfg["nx"])
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and _lib is not None:
-            _lib.orc_destroy(h)
+        L = getattr(self, "L", None)
+        if h and L is not None:
+            L.orc_destroy(h)
             self.h = None
 
     def _check(self, rc):
         if rc != 0:
-            raise OracleError(f"{lib().orc_last_error().decode()} (cell {lib().orc_last_bad_cell()})")
+            raise OracleError(f"{self.L.orc_last_error().decode()} (cell {self.L.orc_last_bad_cell()})")
 
     def zeros(self):
         return np.zeros(self.shape, dtype=np.float64)
 
     def limiter(self, q):
-        self._check(lib().orc_limiter(self.h, _p(q)))
+        self._check(self.L.orc_limiter(self.h, _p(q)))
 
     def glflux(self, q):
-        out = self.zeros(); self._check(lib().orc_glflux(self.h, _p(q), _p(out))); return out
+        out = self.zeros(); self._check(self.L.orc_glflux(self.h, _p(q), _p(out))); return out
 
     def source(self, q):
-        out = self.zeros(); self._check(lib().orc_source(self.h, _p(q), _p(out))); return out
+        out = self.zeros(); self._check(self.L.orc_source(self.h, _p(q), _p(out))); return out
 
     def rhs(self, q):
-        out = self.zeros(); self._check(lib().orc_rhs(self.h, _p(q), _p(out))); return out
+        out = self.zeros(); self._check(self.L.orc_rhs(self.h, _p(q), _p(out))); return out
 
     def euler_step(self, q_in, q_out, dt):
-        self._check(lib().orc_euler_step(self.h, _p(q_in), _p(q_out), float(dt)))
+        self._check(self.L.orc_euler_step(self.h, _p(q_in), _p(q_out), float(dt)))
 
     def min_dt(self, q) -> float:
-        return lib().orc_min_dt(self.h, _p(q))
+        return self.L.orc_min_dt(self.h, _p(q))
 
     def update(self, q_io, q1, q2, dt):
-        self._check(lib().orc_update(self.h, _p(q_io), _p(q1), _p(q2), float(dt)))
+        self._check(self.L.orc_update(self.h, _p(q_io), _p(q1), _p(q2), float(dt)))
 
     def step(self, q_io, q1, q2, t, dt):
         """Reference step semantics: dt from get_min_dt, then RK update; t, dt
         are 0-d float64 arrays updated in place."""
-        self._check(lib().orc_step(self.h, _p(q_io), _p(q1), _p(q2), _p(t), _p(dt)))
+        self._check(self.L.orc_step(self.h, _p(q_io), _p(q1), _p(q2), _p(t), _p(dt)))
 
     def set_step_index(self, n: int):
-        lib().orc_set_step_index(self.h, int(n))
+        self.L.orc_set_step_index(self.h, int(n))
 
     def set_noise_dt(self, dt: float):
-        lib().orc_set_noise_dt(self.h, float(dt))
+        self.L.orc_set_noise_dt(self.h, float(dt))
 
     def set_inflow(self, inflow_def):
         """x-low inflow exterior state, numpy (nQ, nface, nz, ny) C order
         (the reference's Qxlo_ext_def(ny,nz,nface,nQ))."""
         a = np.ascontiguousarray(inflow_def, dtype=np.float64)
         self._inflow = a
-        self._check(lib().orc_set_inflow(self.h, _p(a)))
+        self._check(self.L.orc_set_inflow(self.h, _p(a)))
 
     def set_mask(self, mask, rho_mask: float = 1.25):
         """Solid cells, numpy (nz, ny, nx) bool; None clears."""
         if mask is None:
-            self._check(lib().orc_set_mask(self.h, None, 0.0))
+            self._check(self.L.orc_set_mask(self.h, None, 0.0))
             return
         m = np.ascontiguousarray(mask, dtype=np.uint8)
-        self._check(lib().orc_set_mask(self.h, m.ctypes.data_as(C.c_void_p), float(rho_mask)))
+        self._check(self.L.orc_set_mask(self.h, m.ctypes.data_as(C.c_void_p), float(rho_mask)))
 
     def apply_mask(self, q):
-        self._check(lib().orc_apply_mask(self.h, _p(q)))
+        self._check(self.L.orc_apply_mask(self.h, _p(q)))
 
     def sample(self, q, nvtk: int, fields: int, scales=None) -> np.ndarray:
         """output_vtk's fields: (nsel, nz*nvtk, ny*nvtk, nx*nvtk) float32."""
         nsel = bin(int(fields)).count("1")
         out = np.zeros((nsel, self.shape[2] * nvtk, self.shape[3] * nvtk, self.shape[4] * nvtk), dtype=np.float32)
         sc = None if scales is None else np.ascontiguousarray(scales, dtype=np.float64)
-        self._check(lib().orc_sample(self.h, _p(q), int(nvtk), int(fields),
+        self._check(self.L.orc_sample(self.h, _p(q), int(nvtk), int(fields),
                                      None if sc is None else sc.ctypes.data_as(C.c_void_p),
                                      out.ctypes.data_as(C.c_void_p)))
         return out
@@ -189,14 +203,14 @@ class Oracle:
         out = np.zeros((ids.size, 6))
         row = np.zeros(6)
         for n, i in enumerate(ids):
-            lib().orc_noise_stress(self.h, int(stream), int(i), float(amp_eta), float(amp_zeta), _p(row))
+            self.L.orc_noise_stress(self.h, int(stream), int(i), float(amp_eta), float(amp_zeta), _p(row))
             out[n] = row
         return out
 
     def flux_points(self, qp: np.ndarray, direction: int) -> np.ndarray:
         qp = np.ascontiguousarray(qp, dtype=np.float64)
         out = np.zeros_like(qp)
-        lib().orc_flux_points(self.h, _p(qp), _p(out), int(direction), qp.shape[0])
+        self.L.orc_flux_points(self.h, _p(qp), _p(out), int(direction), qp.shape[0])
         return out
 
     def riemann_face(self, qlr: np.ndarray, direction: int, kind: int) -> np.ndarray:
@@ -205,7 +219,7 @@ class Oracle:
         flr = self.flux_points(qlr, direction)
         nface = qlr.shape[0] // 2
         out = np.zeros((nface, qlr.shape[1]))
-        lib().orc_riemann_face(self.h, _p(qlr), _p(flr), _p(out), int(direction), int(kind))
+        self.L.orc_riemann_face(self.h, _p(qlr), _p(flr), _p(out), int(direction), int(kind))
         return out
 
     def tables(self) -> dict:
@@ -216,7 +230,7 @@ class Oracle:
                  byp=np.zeros((nf, nb)), bym=np.zeros((nf, nb)), bzp=np.zeros((nf, nb)),
                  bzm=np.zeros((nf, nb)), bfaces=np.zeros((6 * nf + npg, nb)))
         keys = ["xq", "w1", "w2", "w3", "cbasis", "bint", "bxp", "bxm", "byp", "bym", "bzp", "bzm", "bfaces"]
-        lib().orc_tables(self.h, *[_p(t[k]) for k in keys])
+        self.L.orc_tables(self.h, *[_p(t[k]) for k in keys])
         return t
 
 

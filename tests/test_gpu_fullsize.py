@@ -89,3 +89,21 @@ def test_ensemble_full_size_conservation(torch_mod):
         assert np.all(np.abs(t1[:5] - a[:5]) <= 1e-12 * scale), (r, (t1[:5] - a[:5]) / scale)
     # keyed noise: realizations of the same size evolve differently
     assert not np.array_equal(outs[0][0, 5], outs[1][0, 5])
+
+
+def test_vortex_context_holds_no_face_flux_arrays(torch_mod):
+    """The fused stage kernel keeps face fluxes on chip: a 128x128x32 nb 10
+    context holds its tables, scalars and the third SSP-RK3 stage array (one
+    state, 210 MB), not the 571 MB of per-direction face-flux arrays that
+    only the table-driven kernels use."""
+    import os
+    from hermeshd_amd.solver import Solver
+    cfg = vortex_3d()
+    assert os.environ.get("HD_KERNELS", "") != "generic"
+    state = 8 * cfg.nbasis * cfg.nQ * cfg.nx * cfg.ny * cfg.nz
+    s = Solver(cfg)
+    assert state <= s.device_bytes < state + (8 << 20), s.device_bytes
+    # the table-driven path still gets its face-flux arrays
+    s2 = Solver(cfg.replace(nx=8, ny=8, nz=8, ieos=2))
+    face_flux_bytes = 8 * cfg.iquad ** 2 * cfg.nQ * 3 * (9 * 8 * 8)
+    assert s2.device_bytes > face_flux_bytes

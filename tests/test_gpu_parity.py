@@ -232,6 +232,9 @@ def test_reference_front_end(torch_mod, oracle_mod):
     assert float(t) == pytest.approx(float(to), rel=1e-13)
     assert int(nout) >= 1
     assert field_rel_err(np.ascontiguousarray(Q.T), q) < TOL
+    # q_1 holds the step's first stage, limited: the reference's Q_r1 after an SSP-RK2 step
+    assert cfg.integrator == "heun"
+    assert field_rel_err(np.ascontiguousarray(Q1.T), qa) < TOL
 
 
 # ------------------------------------------------------- stochastic stress ----

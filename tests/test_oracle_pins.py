@@ -862,3 +862,37 @@ def test_vtr_writer_round_trip(tmp_path):
     np.testing.assert_allclose(back["Z"], np.linspace(-0.5, 0.5, 5), rtol=1e-6)
     text = path.read_text()
     assert 'WholeExtent="0 6 0 4 0 4"' in text
+
+
+# ---------------------------------------------------------------------------
+# viscous relaxation model, boundary branches, quadratic basis (tests/pins.py;
+# test_oracle_mutations.py shows that each fails under a planted mistake)
+# ---------------------------------------------------------------------------
+def test_stress_rows_converge_to_strain_rate(oracle_mod):
+    from tests import pins
+    pins.pin_stress_rows(oracle_mod.Oracle)
+
+
+def test_stress_drives_momentum_and_energy(oracle_mod):
+    from tests import pins
+    pins.pin_stress_into_momentum(oracle_mod.Oracle)
+
+
+def test_relaxation_waves_follow_dispersion_relation(oracle_mod):
+    from tests import pins
+    pins.pin_relaxation_waves(oracle_mod.Oracle)
+
+
+def test_poiseuille_channel_steady_parabola(oracle_mod):
+    from tests import pins
+    pins.pin_poiseuille(oracle_mod.Oracle)
+
+
+def test_boundary_branches_closed_form(oracle_mod):
+    from tests import pins
+    pins.pin_outflow_branches(oracle_mod.Oracle)
+
+
+def test_quadratic_basis_third_order(oracle_mod):
+    from tests import pins
+    pins.pin_nb10_third_order(oracle_mod.Oracle)
