@@ -16,7 +16,7 @@ HD_ERR_UNSUPPORTED = -4
 
 # every extern "C" symbol declared in include/hermeshd_amd.h
 EXPORTED = [
-    "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_update",
+    "hd_create", "hd_destroy", "hd_limit", "hd_rhs", "hd_euler_step", "hd_min_dt", "hd_min_dt_static", "hd_update",
     "hd_step", "hd_stage", "hd_rhs_vjp", "hd_stage_vjp", "hd_set_step", "hd_set_inflow", "hd_set_mask", "hd_apply_mask", "hd_sample", "hd_check", "hd_timing_enable", "hd_timing_read", "hd_device_bytes",
     "hd_last_error",
     "hd_ens_create", "hd_ens_destroy", "hd_ens_state_size", "hd_ens_offsets", "hd_ens_device_bytes",
@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
     L.hd_rhs.argtypes = [vp, vp, vp, vp]
     L.hd_euler_step.argtypes = [vp, vp, vp, vp, vp]
     L.hd_min_dt.argtypes = [vp, vp, vp, vp]
+    L.hd_min_dt_static.argtypes = [C.POINTER(HDConfigC), vp, vp, vp]
     L.hd_update.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hd_step.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.hd_set_step.argtypes = [vp, C.c_uint64, vp]

@@ -95,6 +95,17 @@ int hd_euler_step(hd_ctx *ctx, double *q_in, double *q_out, const double *dt_dev
    (src/hermeshd.f90:145-198), including its sound-speed formula. */
 int hd_min_dt(hd_ctx *ctx, const double *q, double *dt_dev, void *stream);
 
+/* hd_min_dt without a context and without allocating: the time step of the
+   configuration cfg for state q, written to dt_dev.  No counterpart in the
+   reference (get_min_dt, src/hermeshd.f90:145-198, is what it computes); it is
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops).  The workgroup maxima of the reduction live
+   in a static device array of 65536 doubles (512 KiB, held by each GPU the
+   library runs on until the process ends), so grids of more than 16,777,216
+   cells are refused (HD_ERR_INVALID).  Calls that overlap on one GPU share
+   that array and race; calls on one stream do not. */
+int hd_min_dt_static(const hd_config *cfg, const double *q, double *dt_dev, void *stream);
+
 /* One SSP-RK2 or SSP-RK3 step with a given dt.  Replaces RK2/RK3
    (src/integrator.f90:62-86).  q1 and q2 are stage storage as in the
    reference; q2 may be NULL (SSP-RK2 does not need it, and for SSP-RK3 the

@@ -72,3 +72,17 @@ def test_config_validation():
     assert c.shape == (10, 5, 32, 128, 128) and c.cflm == 0.08
     assert abs(1.0 / c.dxi - 10.0 / 128) < 1e-15
     assert sod_1d().cflm == 0.3
+
+
+def test_min_dt_static_refuses_large_grids_before_any_device_call():
+    """The size check of the allocation-free CFL entry point runs on the host,
+    before any HIP call (so it runs here without a GPU): 256x256x257 cells
+    need more workgroup maxima than its static scratch holds."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    cs = _lib.config_struct(HDConfig(nx=256, ny=256, nz=257, nbasis=4, iquad=2, nQ=5).as_ints())
+    dummy = ctypes.c_void_p(8)  # never dereferenced: the call returns before touching it
+    assert L.hd_min_dt_static(ctypes.byref(cs), dummy, dummy, None) == _lib.HD_ERR_INVALID
+    assert b"16777216" in L.hd_last_error()
+    assert L.hd_min_dt_static(None, dummy, dummy, None) == _lib.HD_ERR_INVALID
