@@ -39,6 +39,7 @@ class HDConfigC(C.Structure):
         ("cflm", C.c_double), ("body_force", C.c_double * 3), ("eta_sd", C.c_double), ("zeta_sd", C.c_double),
         ("seed", C.c_uint64), ("realization", C.c_uint32), ("reserved1", C.c_uint32),
         ("eos_p1", C.c_double), ("eos_pbase", C.c_double),
+        ("coll_q", C.c_double), ("col_q", C.c_double),
     ]
 
 

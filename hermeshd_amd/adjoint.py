@@ -40,8 +40,8 @@ class AdjointRunner:
     def __post_init__(self):
         if self.cfg.noise:
             raise ValueError("the adjoint does not differentiate the stochastic stress")
-        if self.cfg.riemann == "roe":
-            raise ValueError("no adjoint for the Roe solver")
+        if self.cfg.ivis == 2 or self.cfg.ieos == 2:
+            raise ValueError("no adjoint for the full 10-moment equations or the water EOS")
         self.order = 3 if self.cfg.integrator == "shu-osher" else 2
         self._solvers = {}
         self.last_stats = {}
@@ -173,4 +173,24 @@ def channel_state(cfg: HDConfig, u0: torch.Tensor, rho: float = 1.0, T: float = 
     P = rho * T
     mode0 = torch.stack([z + rho, rho * u0, z, z, P / (cfg.gamma - 1.0) + 0.5 * rho * u0 * u0] + [z] * (cfg.nQ - 5))
     rest = torch.zeros((cfg.nbasis - 1, cfg.nQ) + tuple(u0.shape), dtype=u0.dtype, device=u0.device)
+    return torch.cat([mode0.unsqueeze(0), rest], dim=0).contiguous()
+
+
+def kinetic_energy(q: torch.Tensor, cfg: HDConfig) -> torch.Tensor:
+    """Kinetic energy of the cell averages, sum over cells of dV |m|^2 / (2 rho)."""
+    dV = (cfg.lx / cfg.nx) * (cfg.ly / cfg.ny) * (cfg.lz / cfg.nz)
+    m0 = q[0]
+    return 0.5 * dV * ((m0[1] ** 2 + m0[2] ** 2 + m0[3] ** 2) / m0[0]).sum()
+
+
+def velocity_state(cfg: HDConfig, vel: torch.Tensor, rho: float = 1.0, p: float | None = None) -> torch.Tensor:
+    """Cell-average state at uniform density and pressure (default 100/gamma,
+    the Taylor-Green vortex) with velocity vel (shape (3, nz, ny, nx)), zero
+    stress and heat flux: differentiable in vel."""
+    p = 100.0 / cfg.gamma if p is None else p
+    z = torch.zeros_like(vel[0])
+    ke = 0.5 * rho * (vel ** 2).sum(0)
+    mode0 = torch.stack([z + rho, rho * vel[0], rho * vel[1], rho * vel[2], p / (cfg.gamma - 1.0) + ke]
+                        + [z] * (cfg.nQ - 5))
+    rest = torch.zeros((cfg.nbasis - 1, cfg.nQ) + tuple(z.shape), dtype=vel.dtype, device=vel.device)
     return torch.cat([mode0.unsqueeze(0), rest], dim=0).contiguous()

@@ -19,6 +19,8 @@ VALID_BASIS = {(4, 2), (8, 2), (10, 3)}
 
 # field indices (0-based) of src/params.f90:42-47
 RH, MX, MY, MZ, EN, PXX, PYY, PZZ, PXY, PXZ, PYZ = range(11)
+# 13-moment model (nQ 13): traceless stress without pzz (= -pxx - pyy), heat flux
+T_PXX, T_PYY, T_PXY, T_PXZ, T_PYZ, T_QX, T_QY, T_QZ = range(5, 13)
 
 
 def reference_T_floor(TK: float = 300.0, mu: float = 18.0) -> float:
@@ -75,17 +77,25 @@ class HDConfig:
     realization: int = 0
     eos_p1: float = field(default_factory=lambda: reference_tait()[0])
     eos_pbase: float = field(default_factory=lambda: reference_tait()[1])
+    # 13-moment model (nQ 13): heat-flux relaxation rate and coll_q * conductivity
+    coll_q: float = 0.0
+    col_q: float = 0.0
 
     def __post_init__(self):
         if (self.nbasis, self.iquad) not in VALID_BASIS:
             raise ValueError(f"(nbasis, iquad) = {(self.nbasis, self.iquad)} not in {sorted(VALID_BASIS)}")
-        if self.nQ not in (5, 11):
-            raise ValueError("nQ must be 5 (hydro fields) or 11 (hydro + stress, reference layout)")
+        if self.nQ not in (5, 11, 13):
+            raise ValueError("nQ must be 5 (hydro fields), 11 (hydro + stress, reference layout) or 13 "
+                             "(hydro + traceless stress + heat flux)")
         if self.ivis not in (0, 1, 2):
             raise ValueError("ivis must be 0 (explicit), 1 (semi-implicit stress relaxation) or 2 "
                              "(full 10-moment equations)")
-        if self.ivis >= 1 and self.nQ != 11:
-            raise ValueError("ivis=1,2 need the stress fields (nQ=11)")
+        if self.ivis >= 1 and self.nQ == 5:
+            raise ValueError("ivis=1,2 need the stress fields (nQ=11 or 13)")
+        if self.nQ == 13 and (self.ivis == 2 or self.noise or self.ieos != 1):
+            raise ValueError("the 13-moment model runs with ivis 0 or 1, an ideal gas and no noise")
+        if self.ivis == 2 and self.nQ != 11:
+            raise ValueError("ivis=2 needs the second moments (nQ=11)")
         if self.ivis == 2 and self.noise:
             raise ValueError("ivis=2 has no stochastic stress")
         if self.noise and self.nQ != 11:
@@ -145,6 +155,7 @@ class HDConfig:
             body_force=list(self.body_force), eta_sd=self.eta_sd, zeta_sd=self.zeta_sd,
             seed=int(self.seed), realization=int(self.realization),
             eos_p1=float(self.eos_p1), eos_pbase=float(self.eos_pbase),
+            coll_q=float(self.coll_q), col_q=float(self.col_q),
         )
 
 
@@ -201,3 +212,23 @@ def fluct_ensemble(vis: float = 0.3, kB: float = 1e-4, sizes=((96, 16), (96, 24)
                     eta_sd=math.sqrt(2.0 * vis * kB), zeta_sd=0.0, seed=seed, realization=0)
     dims = [(n, n, n) for count, n in sizes for _ in range(count)]
     return base, dims
+
+
+def taylor_green_3d(n: int = 192, Re: float = 1600.0, Pr: float = 0.71, gamma: float = 1.4,
+                    riemann: str = "roe") -> HDConfig:
+    """13-moment Taylor-Green vortex: n^3 periodic cells on [0, 2 pi]^3 (the
+    grid is centred on the origin), complete-quadratic basis, rho = 1,
+    p = 100/gamma (Mach 0.1 for |u| = 1), Roe flux, limiter on, SSP-RK3.
+    Viscosity mu = 1/Re (unit velocity and length): the stress relaxes at
+    coll = p/mu with colvis = coll*mu = p; the heat flux relaxes at the same
+    rate, coll_q = coll, with conductivity kappa = mu c_p/Pr, c_p =
+    gamma/(gamma-1), so col_q = coll_q*kappa.  Semi-implicit relaxation
+    (ivis 1): the relaxation times are far below the time step."""
+    L = 2.0 * math.pi
+    p0 = 100.0 / gamma
+    mu = 1.0 / Re
+    coll = p0 / mu
+    kappa = mu * (gamma / (gamma - 1.0)) / Pr
+    return HDConfig(nx=n, ny=n, nz=n, nbasis=10, iquad=3, nQ=13, ivis=1, riemann=riemann, limiter=True,
+                    integrator="shu-osher", bc=("periodic",) * 6, lx=L, ly=L, lz=L, gamma=gamma,
+                    T_floor=1e-4, coll=coll, colvis=coll * mu, coll_q=coll, col_q=coll * kappa)

@@ -31,15 +31,75 @@ namespace hd {
 namespace adj {
 
 // ---------------------------------------------------------------- flux ----
+// reverse of flux_point13<DIR> (the 13-moment flux) from primitives:
+// bF (cotangent of F) -> bQ (+=)
+template <int DIR>
+__device__ __forceinline__ void flux13_vjp(const Phys& ph, const double* Q, const pr::Prim& s, const double* bF,
+                                           double* bQ) {
+    constexpr int ND = MX + DIR;
+    const double un = s.u[DIR];
+    const double P = s.pf;
+    // stress row DIR: indices of T_{DIR,a} (-1: t_zz = -t_xx - t_yy)
+    const int trow[3][3] = {{T_PXX, T_PXY, T_PXZ}, {T_PXY, T_PYY, T_PYZ}, {T_PXZ, T_PYZ, -1}};
+    double Tn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Tn[a] = trow[DIR][a] >= 0 ? Q[trow[DIR][a]] : -(Q[T_PXX] + Q[T_PYY]);
+    double bun = bF[MX] * Q[MX] + bF[MY] * Q[MY] + bF[MZ] * Q[MZ] + bF[EN] * (Q[EN] + P);
+    bun += (DIR == 0 ? ph.c4d3cv : -ph.c2d3cv) * bF[T_PXX] + (DIR == 1 ? ph.c4d3cv : -ph.c2d3cv) * bF[T_PYY];
+    bQ[MX] += bF[MX] * un;
+    bQ[MY] += bF[MY] * un;
+    bQ[MZ] += bF[MZ] * un;
+    bQ[EN] += bF[EN] * un;
+    bQ[ND] += bF[RH];
+    bQ[T_QX + DIR] += bF[EN];
+    // heat-flux row: colq * P / rho
+    const double bq = bF[T_QX + DIR] * ph.colq;
+    const double bP = bF[ND] + bF[EN] * un + bq * s.ri;
+    double bri = bq * P;
+    double bu[3] = {bF[EN] * Tn[0], bF[EN] * Tn[1], bF[EN] * Tn[2]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double bt = bF[MX + a] + bF[EN] * s.u[a];
+        if (trow[DIR][a] >= 0) {
+            bQ[trow[DIR][a]] += bt;
+        } else {
+            bQ[T_PXX] -= bt;
+            bQ[T_PYY] -= bt;
+        }
+    }
+    if (DIR == 0) { bu[1] += ph.colvis * bF[T_PXY]; bu[2] += ph.colvis * bF[T_PXZ]; }
+    if (DIR == 1) { bu[0] += ph.colvis * bF[T_PXY]; bu[2] += ph.colvis * bF[T_PYZ]; }
+    if (DIR == 2) { bu[0] += ph.colvis * bF[T_PXZ]; bu[1] += ph.colvis * bF[T_PYZ]; }
+    bu[DIR] += bun;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        bQ[MX + a] += bu[a] * s.ri;
+        bri += bu[a] * Q[MX + a];
+    }
+    const double smsq = Q[MX] * Q[MX] + Q[MY] * Q[MY] + Q[MZ] * Q[MZ];
+    const double Praw = ph.gm1 * (Q[EN] - 0.5 * s.ri * smsq);
+    if (!(Praw < ph.P_floor)) {
+        bQ[EN] += ph.gm1 * bP;
+        bri += -0.5 * ph.gm1 * smsq * bP;
+#pragma unroll
+        for (int a = 0; a < 3; a++) bQ[MX + a] += -ph.gm1 * s.ri * Q[MX + a] * bP;
+    }
+    bQ[RH] += -s.ri * s.ri * bri;
+}
+
 // reverse of pr::prim + pr::flux<NQ,DIR>: bF (cotangent of F) -> bQ (+=)
 template <int NQ, int DIR>
 __device__ __forceinline__ void flux_vjp(const Phys& ph, const double* Q, const pr::Prim& s, const double* bF,
                                          double* bQ) {
+    if constexpr (NQ == 13) {
+        flux13_vjp<DIR>(ph, Q, s, bF, bQ);
+        return;
+    }
     constexpr int ND = MX + DIR;
     const double un = s.u[DIR];
     double Pn[3] = {0., 0., 0.};
     int pidx[3] = {0, 0, 0};
-    if constexpr (NQ >= 11) {
+    if constexpr (NQ == 11) {
         if (DIR == 0) { pidx[0] = PXX; pidx[1] = PXY; pidx[2] = PXZ; }
         if (DIR == 1) { pidx[0] = PXY; pidx[1] = PYY; pidx[2] = PYZ; }
         if (DIR == 2) { pidx[0] = PXZ; pidx[1] = PYZ; pidx[2] = PZZ; }
@@ -54,7 +114,7 @@ __device__ __forceinline__ void flux_vjp(const Phys& ph, const double* Q, const 
     bQ[ND] += bF[RH];
     const double bP = bF[ND] + bF[EN] * un;
     double bu[3] = {bF[EN] * Pn[0], bF[EN] * Pn[1], bF[EN] * Pn[2]};
-    if constexpr (NQ >= 11) {
+    if constexpr (NQ == 11) {
 #pragma unroll
         for (int a = 0; a < 3; a++) bQ[pidx[a]] += bF[MX + a] + bF[EN] * s.u[a];
         bun += (DIR == 0 ? ph.c4d3cv : -ph.c2d3cv) * bF[PXX] + (DIR == 1 ? ph.c4d3cv : -ph.c2d3cv) * bF[PYY] +
@@ -243,6 +303,13 @@ __device__ __forceinline__ void face_vjp(const Phys& ph, const double* QL, const
             for (int f = 0; f < 5; f++) { bFL[f] = zL[f]; bFR[f] = zR[f]; }
         }
     }
+    if constexpr (RM == RIEMANN_ROE) {
+        // Roe: 0.5 (FL + FR) minus the dissipation of rh..en (or the
+        // Lax-Friedrichs fallback); the reverse adds both parts
+#pragma unroll
+        for (int f = 0; f < 5; f++) { bFL[f] = 0.; bFR[f] = 0.; }
+        roe_point_vjp<DIR>(ph, QL, QR, mu, bQL, bQR, bFL, bFR);
+    }
     flux_vjp<NQ, DIR>(ph, QL, sL, bFL, bQL);
     flux_vjp<NQ, DIR>(ph, QR, sR, bFR, bQR);
 }
@@ -347,6 +414,7 @@ constexpr bool split_fits() { return 3L * NQ * NB * 64 * 8 <= 150L * 1024; }
 struct AdjParams {
     const double* dt;          // device time step
     double coll;
+    double coll_q;             // 13 moments: relaxation rate of the heat flux
     int ivis;
     double a, b;               // out = a*Q^n + b*v
     double bf[3];              // body force
@@ -361,15 +429,18 @@ struct AdjParams {
     const double* inflow;      // x-low inflow exterior state (ny,nz,nface,nQ), or nullptr
 };
 
+// scale * lam through the transposed relaxation: the stress fields by faci,
+// the 13-moment heat flux by faciq
 template <int NB, int NQ>
 __device__ __forceinline__ void load_lp(const double* __restrict__ lam, const Grid& g, long c, double scale,
-                                        double faci, double (&l)[NQ][NB]) {
+                                        double faci, double faciq, double (&l)[NQ][NB]) {
 #pragma unroll
     for (int r = 0; r < NB; r++)
 #pragma unroll
         for (int f = 0; f < NQ; f++) {
             double v = scale * lam[((long)r * NQ + f) * g.N + c];
-            if (f >= PXX) v *= faci;
+            if (NQ == 13 && f >= T_QX) v *= faciq;
+            else if (f >= PXX) v *= faci;
             l[f][r] = v;
         }
 }
@@ -402,7 +473,7 @@ template <int NB, int QP, int NQ, int RM, int DIR, int SIDE>
 __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, const double* __restrict__ lam,
                                               const Grid& g, const fz::CellPos& cp, const double (&m)[NQ][NB],
                                               const double (&lo)[NQ][NB], double (&rt)[NQ][NB], const Phys& ph,
-                                              double dxi_d, double scale, double faci,
+                                              double dxi_d, double scale, double faci, double faciq,
                                               const double* __restrict__ inflow) {
     constexpr int NF = QP * QP;
     const auto& T = fz::face_tab<NB>();
@@ -430,7 +501,7 @@ __device__ __forceinline__ void face_vjp_cell(const double* __restrict__ q, cons
         double nm[NQ][NB], nl[NQ][NB];
         fz::load_modes<NB, NQ>(q, g, cn, nm);
         fz::face_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nm, kn);
-        load_lp<NB, NQ>(lam, g, cn, scale, faci, nl);
+        load_lp<NB, NQ>(lam, g, cn, scale, faci, faciq, nl);
         lift_coeffs<NB, NQ, DIR, 1 - SIDE, NK>(nl, ln);
     }
     bool zero_normal = false;
@@ -573,7 +644,7 @@ template <int NB, int QP, int NQ, int RM>
 __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __restrict__ q,
                                                  const double* __restrict__ lam, const Grid& g, const Phys& ph,
                                                  const AdjParams& ap, const fz::CellPos& cp, double dt, double faci,
-                                                 const double (&m)[NQ][NB], const double (&lo)[NQ][NB],
+                                                 double faciq, const double (&m)[NQ][NB], const double (&lo)[NQ][NB],
                                                  double (&rt)[NQ][NB]) {
     double dcoll = 0.;
     const long c = cp.c;
@@ -582,14 +653,24 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
         fz::fence();
         // sources: ivis 0 explicit relaxation -coll*P; body force rho*g, m.g
         if constexpr (NQ >= 11) {
+            // the stress fields (and, with 13 moments, the heat flux at its own rate)
+            constexpr int SLAST = NQ == 13 ? T_PYZ : PYZ;
             if (ap.ivis == 0 && ap.coll != 0.) {
 #pragma unroll
-                for (int f = PXX; f <= PYZ; f++)
+                for (int f = PXX; f <= SLAST; f++)
 #pragma unroll
                     for (int r = 0; r < NB; r++) {
                         rt[f][r] += -ap.coll * lo[f][r];
                         dcoll += -dt * lo[f][r] * m[f][r];
                     }
+            }
+            if constexpr (NQ == 13) {
+                if (ap.ivis == 0 && ap.coll_q != 0.) {
+#pragma unroll
+                    for (int f = T_QX; f <= T_QZ; f++)
+#pragma unroll
+                        for (int r = 0; r < NB; r++) rt[f][r] += -ap.coll_q * lo[f][r];
+                }
             }
             if (ap.ivis == 1 && ap.coll != 0. && ap.out) {
                 // v = faci*(m + dt*R): dv/dcoll = -dt*faci*v, b*v = out - a*Q^n;
@@ -597,7 +678,7 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
                 // loads are issued together
                 if (ap.a != 0.) {
 #pragma unroll
-                    for (int f = PXX; f <= PYZ; f++)
+                    for (int f = PXX; f <= SLAST; f++)
 #pragma unroll
                         for (int r = 0; r < NB; r++) {
                             const long o = ((long)r * NQ + f) * g.N + c;
@@ -606,7 +687,7 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
                         }
                 } else {
 #pragma unroll
-                    for (int f = PXX; f <= PYZ; f++)
+                    for (int f = PXX; f <= SLAST; f++)
 #pragma unroll
                         for (int r = 0; r < NB; r++) {
                             const long o = ((long)r * NQ + f) * g.N + c;
@@ -627,21 +708,21 @@ __device__ __forceinline__ double cell_vjp_parts(int parts, const double* __rest
         }
     }
     if (parts & 2) {
-        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 0, 0>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 0, 1>(q, lam, g, cp, m, lo, rt, ph, g.dxi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
     }
     if (parts & 4) {
-        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 1, 0>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 1, 1>(q, lam, g, cp, m, lo, rt, ph, g.dyi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
     }
     if (parts & 8) {
-        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 2, 0>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
-        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, ap.inflow);
+        face_vjp_cell<NB, QP, NQ, RM, 2, 1>(q, lam, g, cp, m, lo, rt, ph, g.dzi, ap.b, faci, faciq, ap.inflow);
         fz::fence();
     }
     return dcoll;
@@ -719,14 +800,15 @@ __global__ __launch_bounds__(256) void k_stage_vjp(const double* __restrict__ q,
         const fz::CellPos cp = cell_pos(g, c);
         const double dt = *ap.dt;
         const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+        const double faciq = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll_q) : 1.0;
         double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
         fz::load_modes<NB, NQ>(q, g, c, m);
-        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
+        load_lp<NB, NQ>(lam, g, c, ap.b, faci, faciq, lo);
 #pragma unroll
         for (int f = 0; f < NQ; f++)
 #pragma unroll
             for (int r = 0; r < NB; r++) rt[f][r] = 0.;
-        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(15, q, lam, g, ph, ap, cp, dt, faci, m, lo, rt);
+        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(15, q, lam, g, ph, ap, cp, dt, faci, faciq, m, lo, rt);
         cell_vjp_finish<NB, QP, NQ>(lam_in, g, ap, c, dt, lo, rt);
     }
     if (ap.partial) block_partial(dcoll, ap.partial);
@@ -747,6 +829,7 @@ __global__ __launch_bounds__(256) void k_stage_vjp_split(const double* __restric
     double dcoll = 0.;
     const double dt = *ap.dt;
     const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+    const double faciq = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll_q) : 1.0;
     double m[NQ][NB], lo[NQ][NB], rt[NQ][NB];
 #pragma unroll
     for (int f = 0; f < NQ; f++)
@@ -757,8 +840,8 @@ __global__ __launch_bounds__(256) void k_stage_vjp_split(const double* __restric
     if (valid) {
         cp = cell_pos(g, c);
         fz::load_modes<NB, NQ>(q, g, c, m);
-        load_lp<NB, NQ>(lam, g, c, ap.b, faci, lo);
-        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(1 << wave, q, lam, g, ph, ap, cp, dt, faci, m, lo, rt);
+        load_lp<NB, NQ>(lam, g, c, ap.b, faci, faciq, lo);
+        dcoll = cell_vjp_parts<NB, QP, NQ, RM>(1 << wave, q, lam, g, ph, ap, cp, dt, faci, faciq, m, lo, rt);
     }
     if (wave > 0) {
 #pragma unroll

@@ -32,6 +32,7 @@ bool pick_ops_for(int nb, int q, int nq, int rm, Ops* o) {
     if (nb == 8 && q == 2) return hd_pick_8(nq, rm, o);
     if (nb == 10 && q == 3 && nq == 5) return hd_pick_10_5(rm, o);
     if (nb == 10 && q == 3 && nq == 11) return hd_pick_10_11(rm, o);
+    if (nb == 10 && q == 3 && nq == 13) return hd_pick_10_13(rm, o);
     return false;
 }
 
@@ -41,9 +42,12 @@ int validate(const hd_config* c) {
     const bool ok = (c->nbasis == 4 && c->iquad == 2) || (c->nbasis == 8 && c->iquad == 2) ||
                     (c->nbasis == 10 && c->iquad == 3);
     if (!ok) return fail(HD_ERR_INVALID, "(nbasis, iquad) must be (4,2), (8,2) or (10,3)");
-    if (c->nQ != 5 && c->nQ != 11) return fail(HD_ERR_INVALID, "nQ must be 5 or 11");
+    if (c->nQ != 5 && c->nQ != 11 && c->nQ != 13) return fail(HD_ERR_INVALID, "nQ must be 5, 11 or 13");
     if (c->ivis < 0 || c->ivis > 2) return fail(HD_ERR_INVALID, "ivis must be 0, 1 or 2");
-    if (c->ivis >= 1 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=1,2 need nQ=11");
+    if (c->ivis >= 1 && c->nQ == 5) return fail(HD_ERR_INVALID, "ivis=1,2 need the stress fields (nQ=11 or 13)");
+    if (c->ivis == 2 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=2 needs the second moments (nQ=11)");
+    if (c->nQ == 13 && (c->noise || c->ieos != 1))
+        return fail(HD_ERR_INVALID, "the 13-moment model runs with an ideal gas and no noise");
     if (c->ivis == 2 && c->noise) return fail(HD_ERR_INVALID, "ivis=2 has no stochastic stress");
     if (c->ieos != 1 && c->ieos != 2) return fail(HD_ERR_INVALID, "ieos must be 1 (ideal gas) or 2 (water)");
     if (c->ieos == 2 && c->riemann == HD_RIEMANN_ROE)
@@ -92,6 +96,7 @@ StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int
     StageParams sp;
     sp.dt = dt_dev;
     sp.coll = c->cfg.coll;
+    sp.coll_q = c->cfg.coll_q;
     sp.ivis = c->cfg.ivis;
     sp.combine = combine;
     sp.mode = mode;
@@ -244,6 +249,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     ph.colvis = cfg->colvis; ph.c2d3cv = (2. / 3.) * cfg->colvis; ph.c4d3cv = (4. / 3.) * cfg->colvis;
     ph.ivis = cfg->ivis;
     ph.ieos = cfg->ieos; ph.P1 = cfg->eos_p1; ph.Pbase = cfg->eos_pbase;
+    ph.colq = cfg->col_q;
     c->lp.rh_floor = cfg->rh_floor; c->lp.epsP = cfg->rh_floor * cfg->T_floor; c->lp.gm1 = ph.gm1;
     build_tables(c->nb, c->q, &c->htab);
     int r = HD_OK;
@@ -254,7 +260,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
     // table-driven kernels only; HD_KERNELS=generic selects them for any configuration
     const char* kern = std::getenv("HD_KERNELS");
-    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2;
+    c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2 ||
+                 cfg->nQ == 13;
     if (c->generic && cfg->noise) {
         hd_destroy(c);
         return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
@@ -488,14 +495,16 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     if (c->cfg.noise) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the stochastic stress");
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
-    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
-    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
+    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "configuration has no adjoint");
+    if (c->cfg.ivis == 2 || c->cfg.ieos == 2)
+        return fail(HD_ERR_UNSUPPORTED, "no adjoint for the full 10-moment equations or the water EOS");
     if (c->mask_on) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the immersed-solid mask");
     if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
         return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     adj::AdjParams ap;
     ap.dt = dt_dev;
     ap.coll = c->cfg.coll;
+    ap.coll_q = c->cfg.coll_q;
     ap.ivis = c->cfg.ivis;
     ap.a = args->a;
     ap.b = args->b;
@@ -527,13 +536,15 @@ int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void*
     if (c->cfg.noise) return fail(HD_ERR_UNSUPPORTED, "the adjoint does not differentiate the stochastic stress");
     Ops o;
     if (!pick_ops(c, &o)) return fail(HD_ERR_UNSUPPORTED, "configuration not built");
-    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "no adjoint for the Roe solver");
-    if (c->generic) return fail(HD_ERR_UNSUPPORTED, "the adjoint runs with the fused kernels (ivis 0 or 1)");
+    if (!o.stage_vjp) return fail(HD_ERR_UNSUPPORTED, "configuration has no adjoint");
+    if (c->cfg.ivis == 2 || c->cfg.ieos == 2)
+        return fail(HD_ERR_UNSUPPORTED, "no adjoint for the full 10-moment equations or the water EOS");
     if (c->cfg.bc[0] == HD_BC_INFLOW && !c->inflow_set)
         return fail(HD_ERR_INVALID, "inflow boundary without its state (hd_set_inflow)");
     adj::AdjParams ap{};
     ap.dt = c->scratch_dt;
     ap.coll = c->cfg.coll;
+    ap.coll_q = c->cfg.coll_q;
     ap.ivis = c->cfg.ivis;
     ap.a = 0.;
     ap.b = 1.;

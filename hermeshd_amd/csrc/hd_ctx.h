@@ -4,7 +4,7 @@
 // (the reference keeps these as Fortran module globals; src/params.f90,
 // src/basis_funcs.f90, src/boundary.f90, src/flux.f90:11-18).  The kernels of
 // each (nbasis, iquad) family are instantiated in their own translation unit
-// (hd_k4.hip, hd_k8.hip, hd_k10_5.hip, hd_k10_11.hip) so that they compile in
+// (hd_k4.hip, hd_k8.hip, hd_k10_5.hip, hd_k10_11.hip, hd_k10_13.hip) so that they compile in
 // parallel; hd_pick_* fills the dispatch table for one configuration.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -137,6 +137,7 @@ bool hd_pick_4(int nq, int rm, Ops* o);
 bool hd_pick_8(int nq, int rm, Ops* o);
 bool hd_pick_10_5(int rm, Ops* o);
 bool hd_pick_10_11(int rm, Ops* o);
+bool hd_pick_10_13(int rm, Ops* o);
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

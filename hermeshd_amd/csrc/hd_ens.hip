@@ -102,6 +102,7 @@ StageParams ens_params(const hd_ens* e, const double* dt_dev, int combine, int m
     StageParams sp;
     sp.dt = dt_dev;
     sp.coll = e->base.coll;
+    sp.coll_q = e->base.coll_q;
     sp.ivis = e->base.ivis;
     sp.combine = combine;
     sp.mode = mode;
@@ -207,8 +208,8 @@ int hd_ens_create(const hd_config* base, int32_t nreal, const int32_t* dims, con
     if (nreal < 1) return fail(HD_ERR_INVALID, "nreal must be >= 1");
     int rc = validate(base);
     if (rc) return rc;
-    if (base->ivis == 2 || base->ieos == 2)
-        return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (ivis 0 or 1, ieos 1)");
+    if (base->ivis == 2 || base->ieos == 2 || base->nQ == 13)
+        return fail(HD_ERR_UNSUPPORTED, "ensembles run the fused kernels (nQ 5 or 11, ivis 0 or 1, ieos 1)");
     if (base->bc[0] == HD_BC_INFLOW) return fail(HD_ERR_UNSUPPORTED, "ensembles take no inflow boundary");
     for (int r = 0; r < nreal; r++) {
         hd_config c = *base;

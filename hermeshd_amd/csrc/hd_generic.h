@@ -167,7 +167,7 @@ __device__ __forceinline__ void apply_bc(int kind, int dir, int side, const doub
 // physical flux of the configured equations at one point
 template <int NQ, int DIR>
 __device__ __forceinline__ void gflux(const Phys& ph, const double* Q, double* F) {
-    if constexpr (NQ >= 11) {
+    if constexpr (NQ == 11) {
         if (ph.ivis == 2) {
             flux_point10<DIR>(ph, Q, F);
             return;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q,
         gflux<NQ, DIR>(ph, QR[p], FR);
 #pragma unroll
         for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
-        if constexpr (NQ >= 11) {
+        if constexpr (NQ == 11) {
             // ivis 2: local Lax-Friedrichs dissipation on every field with the
             // 10-moment freezing speeds (src/flux.f90:243-263)
             if (ph.ivis == 2) {
@@ -266,6 +266,7 @@ __global__ __launch_bounds__(128) void k_face_flux(const double* __restrict__ q,
 struct StageParams {
     const double* dt;     // device scalar
     double coll;          // relaxation (ivis 1: implicit factor; ivis 0: explicit source)
+    double coll_q;        // 13 moments: relaxation rate of the heat flux
     int ivis;
     int combine;          // 0: none, 1: 0.5*(qn+qs), 2: 0.75*qn+0.25*qs, 3: qn/3+2qs/3
     int mode;             // 0: stage update, 1: write rhs = -glflux+source
@@ -354,6 +355,7 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
     const long fxp = fxm + 1, fyp = fym + g.nx, fzp = fzm + (long)g.nx * g.ny;
     const double dt = *sp.dt;
     const double faci = 1. / (1. + dt * sp.coll);
+    const double faciq = 1. / (1. + dt * sp.coll_q);
     bool bad = false;
     // NaN check of mode 1, SSP combination, store
     auto emit = [&](int f, int r, double v) {
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
     };
     // ivis 2: glflux and source of the diagonal second moments, kept for the
     // trace-coupled implicit update
-    constexpr int ND = NQ >= 11 ? 3 : 1;
+    constexpr int ND = NQ == 11 ? 3 : 1;
     double dglf[ND][NB], dsrc[ND][NB];
 #pragma unroll
     for (int f = 0; f < NQ; f++) {
@@ -453,14 +455,16 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 #pragma unroll
                 for (int r = 0; r < NB; r++) src[r] = 0.125 * T->cb[r] * s[r];
             }
-            if (sp.ivis == 0 && f >= PXX && sp.coll != 0.) {
+            // 13 moments: the heat flux relaxes at its own rate
+            const double rate = (NQ == 13 && f >= T_QX) ? sp.coll_q : sp.coll;
+            if (sp.ivis == 0 && f >= PXX && rate != 0.) {
                 for (int r = 0; r < NB; r++) {
                     double s = 0.;
                     for (int p = 0; p < NPG; p++) {
                         double qv = 0.;
 #pragma unroll
                         for (int rr = 0; rr < NB; rr++) qv += T->bint[p][rr] * m[f][rr];
-                        s += T->bwint[p][r] * (-sp.coll * qv);
+                        s += T->bwint[p][r] * (-rate * qv);
                     }
                     src[r] = 0.125 * T->cb[r] * s;
                 }
@@ -478,7 +482,7 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
             for (int r = 0; r < NB; r++) qout[qoff(g, c, f, r)] = -(glf[r] - src[r]);
             continue;
         }
-        if constexpr (NQ >= 11) {
+        if constexpr (NQ == 11) {
             if (sp.ivis == 2 && f >= PXX) {
                 if (f <= PZZ) {
 #pragma unroll
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 #pragma unroll
         for (int r = 0; r < NB; r++) {
             double v = m[f][r] - dt * (glf[r] - src[r]);
-            if (sp.ivis == 1 && f >= PXX) v = faci * v;
+            if (sp.ivis == 1 && f >= PXX) v = (NQ == 13 && f >= T_QX ? faciq : faci) * v;
             emit(f, r, v);
         }
     }

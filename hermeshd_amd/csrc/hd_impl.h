@@ -69,7 +69,7 @@ struct Impl {
     }
     static void stage_vjp(hd_ctx* c, const double* q, const double* lam, double* lam_in, const adj::AdjParams& ap,
                           hipStream_t s) {
-        if constexpr (RM != RIEMANN_ROE) {
+        {
             // small grids are latency-bound: spread each cell over four waves
             if constexpr (adj::split_fits<NB, NQ>()) {
                 if (c->N <= adj::kSplitCells) {
@@ -85,7 +85,10 @@ struct Impl {
         }
     }
     static Ops ops() {
-        return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage, RM == RIEMANN_ROE ? nullptr : &stage_vjp};
+        // the 13-moment model runs forward on the table-driven kernels (its
+        // transpose is the fused adjoint kernel); no ensembles
+        if constexpr (NQ == 13) return Ops{&limiter, &faces, &cell, nullptr, nullptr, nullptr, &stage_vjp};
+        else return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage, &stage_vjp};
     }
 };
 

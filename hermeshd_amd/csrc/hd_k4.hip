@@ -4,6 +4,7 @@ namespace hd {
 bool hd_pick_4(int nq, int rm, Ops* o) {
     if (nq == 5) return pick_rm<4, 2, 5>(rm, o);
     if (nq == 11) return pick_rm<4, 2, 11>(rm, o);
+    if (nq == 13) return pick_rm<4, 2, 13>(rm, o);
     return false;
 }
 }  // namespace hd

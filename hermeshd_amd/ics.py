@@ -137,7 +137,7 @@ def uniform(cfg: HDConfig, rho: float = 1.0, T: float = 1.0, v=(0.0, 0.0, 0.0)) 
 def taylor_green(cfg: HDConfig, rho: float = 1.0, noise: float = 1e-4, seed: int = 1234) -> np.ndarray:
     """Taylor-Green vortex on [0, L]^3 (coordinates shifted from the centred
     grid): u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
-    p = 100/gamma, plus noise*N(0,1) on u and v."""
+    p = 100/gamma, plus noise*N(0,1) on each velocity component."""
     q = background(cfg)
     xc, yc, zc = cell_centres(cfg)
     x = (xc + cfg.lx / 2)[None, None, :]
@@ -147,7 +147,7 @@ def taylor_green(cfg: HDConfig, rho: float = 1.0, noise: float = 1e-4, seed: int
     rng = np.random.Generator(np.random.PCG64(seed))
     u = np.sin(x) * np.cos(y) * np.cos(z) + noise * rng.standard_normal(shp)
     v = -np.cos(x) * np.sin(y) * np.cos(z) + noise * rng.standard_normal(shp)
-    w = np.zeros(shp)
+    w = noise * rng.standard_normal(shp)
     _set_primitive(q, cfg, np.full(shp, rho), u, v, w, np.full(shp, 100.0 / cfg.gamma))
     return q
 

@@ -44,7 +44,11 @@ extern "C" {
 /* The compile-time switches of src/input.f90 and src/params.f90, at run time. */
 typedef struct hd_config {
     int32_t nx, ny, nz;          /* cells (one device holds the whole grid) */
-    int32_t nbasis, iquad, nQ;   /* (4,2), (8,2) or (10,3); nQ 5 or 11 */
+    int32_t nbasis, iquad, nQ;   /* (4,2), (8,2) or (10,3); nQ 5, 11 or 13.  nQ 13 is the
+                                    13-moment model (no counterpart in the reference): fields
+                                    rh,mx,my,mz,en,pxx,pyy,pxy,pxz,pyz,qx,qy,qz, traceless stress
+                                    (pzz = -pxx-pyy) and Maxwell-Cattaneo heat flux; ivis 0 or 1,
+                                    ideal gas, no noise */
     int32_t ivis, ieos;          /* ivis 0 explicit, 1 semi-implicit stress relaxation, 2 full
                                     10-moment equations (nQ 11, no noise, no adjoint or ensemble:
                                     table-driven kernels); ieos 1 ideal gas, 2 Murnaghan-Tait
@@ -66,6 +70,8 @@ typedef struct hd_config {
     uint32_t realization, reserved1;
     double eos_p1, eos_pbase;    /* ieos 2: P = eos_p1*(rho^7.2 - 1) + eos_pbase + P_ideal
                                     (src/params.f90:97-105: P_1, P_base) */
+    double coll_q, col_q;        /* nQ 13: heat-flux relaxation rate and coll_q*conductivity
+                                    (no counterpart in the reference) */
 } hd_config;
 
 typedef struct hd_ctx hd_ctx;

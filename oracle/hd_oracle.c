@@ -25,6 +25,9 @@
  */
 #include "hd_oracle.h"
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,10 +36,13 @@
 #define QMAX 3
 #define NFMAX (QMAX * QMAX)
 #define NPGMAX (QMAX * QMAX * QMAX)
-#define NQMAX 11
+#define NQMAX 13
 
 /* field indices, 0-based (src/params.f90:42-47) */
 enum { RH = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PXX = 5, PYY = 6, PZZ = 7, PXY = 8, PXZ = 9, PYZ = 10 };
+/* 13-moment fields (no counterpart in the reference): traceless stress
+   without pzz (= -pxx - pyy) and the heat flux */
+enum { T_PXX = 5, T_PYY = 6, T_PXY = 7, T_PXZ = 8, T_PYZ = 9, T_QX = 10, T_QY = 11, T_QZ = 12 };
 /* mode indices, 0-based (src/initialize.f90:126-194 with ibitri = 0 or nbasis = 8) */
 enum { K1 = 0, KX = 1, KY = 2, KZ = 3, KYZ = 4, KZX = 5, KXY = 6, KXYZ = 7, KXX = 7, KYY = 8, KZZ = 9 };
 
@@ -48,6 +54,18 @@ static int set_err(int code, const char *msg) {
     return code;
 }
 const char *orc_last_error(void) { return g_err; }
+
+/* threads of the cell and face loops (OpenMP builds; 1 otherwise): results do
+   not depend on it, since every element is computed by the same code */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+    return n > 0 ? n : 1;
+#else
+    (void)n;
+    return 1;
+#endif
+}
 int64_t orc_last_bad_cell(void) { return g_bad_cell; }
 
 struct orc_ctx {
@@ -173,9 +191,13 @@ orc_ctx *orc_create(const orc_config *cfg) {
     int ok = (cfg->nbasis == 4 && cfg->iquad == 2) || (cfg->nbasis == 8 && cfg->iquad == 2) ||
              (cfg->nbasis == 10 && cfg->iquad == 3);
     if (!ok) { set_err(-1, "unsupported (nbasis, iquad)"); return NULL; }
-    if (cfg->nQ != 5 && cfg->nQ != 11) { set_err(-1, "nQ must be 5 or 11"); return NULL; }
+    if (cfg->nQ != 5 && cfg->nQ != 11 && cfg->nQ != 13) { set_err(-1, "nQ must be 5, 11 or 13"); return NULL; }
     if (cfg->ivis < 0 || cfg->ivis > 2) { set_err(-1, "ivis must be 0, 1 or 2"); return NULL; }
-    if (cfg->ivis >= 1 && cfg->nQ != 11) { set_err(-1, "ivis=1,2 need nQ=11"); return NULL; }
+    if (cfg->ivis >= 1 && cfg->nQ == 5) { set_err(-1, "ivis=1,2 need the stress fields"); return NULL; }
+    if (cfg->nQ == 13 && (cfg->ivis == 2 || cfg->noise || cfg->ieos != 1)) {
+        set_err(-1, "the 13-moment model runs with ivis 0 or 1, an ideal gas and no noise");
+        return NULL;
+    }
     if (cfg->ivis == 2 && cfg->noise) { set_err(-1, "ivis=2 has no stochastic stress"); return NULL; }
     if (cfg->noise && cfg->nQ != 11) { set_err(-1, "noise needs nQ=11"); return NULL; }
     if (cfg->ieos != 1 && cfg->ieos != 2) { set_err(-1, "ieos must be 1 or 2"); return NULL; }
@@ -348,7 +370,7 @@ static void flux_points_s(const orc_ctx *c, const double *qp, double *fp, int di
         double *F = fp + (long)p * nQ;
         double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
         double P_xx = 0, P_yy = 0, P_zz = 0, P_xy = 0, P_xz = 0, P_yz = 0;
-        if (nQ >= 11) { P_xx = Q[PXX]; P_yy = Q[PYY]; P_zz = Q[PZZ]; P_xy = Q[PXY]; P_xz = Q[PXZ]; P_yz = Q[PYZ]; }
+        if (nQ == 11) { P_xx = Q[PXX]; P_yy = Q[PYY]; P_zz = Q[PZZ]; P_xy = Q[PXY]; P_xz = Q[PXZ]; P_yz = Q[PYZ]; }
         double Sxx = 0, Syy = 0, Szz = 0, Sxy = 0, Sxz = 0, Syz = 0;
         if (S) { Sxx = S[6 * p + 0]; Syy = S[6 * p + 1]; Szz = S[6 * p + 2]; Sxy = S[6 * p + 3]; Sxz = S[6 * p + 4]; Syz = S[6 * p + 5]; }
         double dni = 1. / dn;
@@ -358,6 +380,49 @@ static void flux_points_s(const orc_ctx *c, const double *qp, double *fp, int di
         /* Murnaghan-Tait water (src/flux.f90:64, src/params.f90:97-105) */
         if (c->c.ieos == 2) P = c->c.eos_p1 * (pow(dn, 7.2) - 1.) + c->c.eos_pbase + P;
         if (P < c->P_floor) P = c->P_floor;
+        if (nQ == 13) {
+            /* 13-moment relaxation model (no counterpart in the reference): the
+               stress rows of the ivis 0/1 model (src/flux.f90:101-106,138-143,
+               175-180) for the five stored components, pzz = -pxx - pyy, and a
+               Maxwell-Cattaneo heat flux, d_t q_i + d_i(col_q T) = -coll_q q_i
+               with T = P/rho, whose q_k enters the energy flux */
+            const double T_xx = Q[T_PXX], T_yy = Q[T_PYY], T_zz = -(T_xx + T_yy);
+            const double T_xy = Q[T_PXY], T_xz = Q[T_PXZ], T_yz = Q[T_PYZ];
+            const double ct = c->c.col_q * (P * dni);
+            switch (dir) {
+            case 1:
+                F[RH] = M_x;
+                F[MX] = M_x * vx + P + T_xx;
+                F[MY] = M_y * vx + T_xy;
+                F[MZ] = M_z * vx + T_xz;
+                F[EN] = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz + Q[T_QX];
+                F[T_PXX] = c4d3cv * vx; F[T_PYY] = -c2d3cv * vx;
+                F[T_PXY] = colvis * vy; F[T_PXZ] = colvis * vz; F[T_PYZ] = 0;
+                F[T_QX] = ct; F[T_QY] = 0; F[T_QZ] = 0;
+                break;
+            case 2:
+                F[RH] = M_y;
+                F[MX] = M_x * vy + T_xy;
+                F[MY] = M_y * vy + P + T_yy;
+                F[MZ] = M_z * vy + T_yz;
+                F[EN] = (Ener + P) * vy + T_yy * vy + T_xy * vx + T_yz * vz + Q[T_QY];
+                F[T_PXX] = -c2d3cv * vy; F[T_PYY] = c4d3cv * vy;
+                F[T_PXY] = colvis * vx; F[T_PXZ] = 0; F[T_PYZ] = colvis * vz;
+                F[T_QX] = 0; F[T_QY] = ct; F[T_QZ] = 0;
+                break;
+            default:
+                F[RH] = M_z;
+                F[MX] = M_x * vz + T_xz;
+                F[MY] = M_y * vz + T_yz;
+                F[MZ] = M_z * vz + P + T_zz;
+                F[EN] = (Ener + P) * vz + T_zz * vz + T_xz * vx + T_yz * vy + Q[T_QZ];
+                F[T_PXX] = -c2d3cv * vz; F[T_PYY] = -c2d3cv * vz;
+                F[T_PXY] = 0; F[T_PXZ] = colvis * vx; F[T_PYZ] = colvis * vy;
+                F[T_QX] = 0; F[T_QY] = 0; F[T_QZ] = ct;
+                break;
+            }
+            continue;
+        }
         if (c->c.ivis == 2) {
             /* full 10-moment model (src/flux.f90:64-66,89-117,134-155,172-191):
                the stress fields hold the second moments E_ij = rho u_i u_j + P_ij;
@@ -676,6 +741,7 @@ int orc_limiter(orc_ctx *c, double *q) {
     const orc_config *g = &c->c;
     const int nb = c->nb, npge = c->npge;
     const double epsi = g->rh_floor, epsiP = g->rh_floor * g->T_floor, aindm1 = c->aindm1;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < g->nz; k++)
         for (int j = 0; j < g->ny; j++)
             for (int i = 0; i < g->nx; i++) {
@@ -807,7 +873,10 @@ int orc_sample(const orc_ctx *c, const double *q, int nvtk, uint32_t fields, con
                     case 9: val = P * sc; break;
                     case 10: val = nQ > PXX ? v[PXX] * sc : 0.; break;
                     case 11: val = nQ > PYY ? v[PYY] * sc : 0.; break;
-                    case 12: val = nQ > PZZ ? v[PZZ] * sc : 0.; break;
+                    case 12:
+                        /* the 13-moment model stores the traceless stress without pzz */
+                        val = nQ == 13 ? -(v[T_PXX] + v[T_PYY]) * sc : (nQ > PZZ ? v[PZZ] * sc : 0.);
+                        break;
                     case 13: val = -2. * (dxvy - dyvx) * c->dxi * sc; break;
                     }
                     out[(long)s * npts + id] = (float)val;
@@ -954,6 +1023,7 @@ static void calc_face_fluxes(orc_ctx *c, const double *q, double noise_amp_eta, 
     for (int dir = 1; dir <= 3; dir++) {
         int ni = nx + (dir == 1), nj = ny + (dir == 2), nk = nz + (dir == 3);
         double *out = dir == 1 ? c->fx : dir == 2 ? c->fy : c->fz;
+#pragma omp parallel for schedule(static) private(Qf, Ff, Fo, S)
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
@@ -1015,6 +1085,7 @@ static void inner_integral(orc_ctx *c, const double *q, double noise_amp_eta, do
     double Qin[NPGMAX * NQMAX], fx[NPGMAX * NQMAX], fy[NPGMAX * NQMAX], fz[NPGMAX * NQMAX], S[NPGMAX * 6];
     const double dxi = c->dxi, dyi = c->dyi, dzi = c->dzi;
     const double *w = c->w3;
+#pragma omp parallel for schedule(static) private(Qin, fx, fy, fz, S)
     for (int k = 0; k < g->nz; k++)
         for (int j = 0; j < g->ny; j++)
             for (int i = 0; i < g->nx; i++) {
@@ -1119,6 +1190,7 @@ int orc_glflux(orc_ctx *c, const double *q, double *glflux) {
     calc_face_fluxes(c, q, ae, az);
     inner_integral(c, q, aev, azv);
     const double *w2 = c->w2;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int f = 0; f < nQ; f++)
         for (int k = 0; k < g->nz; k++)
             for (int j = 0; j < g->ny; j++)
@@ -1148,6 +1220,7 @@ int orc_source(orc_ctx *c, const double *q, double *source) {
     const orc_config *g = &c->c;
     const int nb = c->nb, npg = c->npg, nQ = c->nQ;
     double src[NPGMAX * NQMAX];
+#pragma omp parallel for schedule(static) private(src)
     for (int k = 0; k < g->nz; k++)
         for (int j = 0; j < g->ny; j++)
             for (int i = 0; i < g->nx; i++) {
@@ -1159,8 +1232,13 @@ int orc_source(orc_ctx *c, const double *q, double *source) {
                         Qin[f] = s;
                     }
                     for (int f = 0; f < nQ; f++) src[p * nQ + f] = 0;
-                    if (nQ >= 11 && g->ivis == 0)
+                    if (nQ == 11 && g->ivis == 0)
                         for (int f = PXX; f <= PYZ; f++) src[p * nQ + f] = -g->coll * Qin[f];
+                    if (nQ == 13 && g->ivis == 0) {
+                        /* explicit relaxation of the stress and the heat flux */
+                        for (int f = T_PXX; f <= T_PYZ; f++) src[p * nQ + f] = -g->coll * Qin[f];
+                        for (int f = T_QX; f <= T_QZ; f++) src[p * nQ + f] = -g->coll_q * Qin[f];
+                    }
                     if (nQ >= 11 && g->ivis == 2) {
                         /* explicit 10-moment relaxation source (src/sources.f90:55-73) */
                         double dn = Qin[RH], dni = 1. / dn;
@@ -1213,13 +1291,17 @@ int orc_euler_step(orc_ctx *c, double *q_in, double *q_out, double dt) {
     if (rc) return rc;
     orc_source(c, q_in, c->src);
     double faci = 1. / (1. + dt * g->coll);
+    double faciq = 1. / (1. + dt * g->coll_q);
     long n = c->ncell;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int r = 0; r < nb; r++)
         for (int f = 0; f < nQ; f++)
             for (long e = 0; e < n; e++) {
                 long o = e + n * (f + (long)nQ * r);
                 double v = q_in[o] - dt * (c->glf[o] - c->src[o]);
-                if (g->ivis == 1 && f >= PXX) v = faci * v;
+                /* 13-moment: the heat flux relaxes at its own rate */
+                if (g->ivis == 1 && nQ == 13 && f >= T_QX) v = faciq * v;
+                else if (g->ivis == 1 && f >= PXX) v = faci * v;
                 q_out[o] = v;
             }
     if (g->ivis == 2) {

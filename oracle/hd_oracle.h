@@ -8,6 +8,9 @@
  * State layout: Q(nx,ny,nz,nQ,nbasis) in Fortran order, i.e. a C array
  * q[r][f][k][j][i] with i fastest (reference src/params.f90:163).
  * Field order rh,mx,my,mz,en[,pxx,pyy,pzz,pxy,pxz,pyz] (src/params.f90:42-47).
+ * The 13-moment model (nQ = 13, no counterpart in the reference) orders its
+ * fields rh,mx,my,mz,en,pxx,pyy,pxy,pxz,pyz,qx,qy,qz: the traceless stress
+ * (pzz = -pxx - pyy) and the heat flux.
  */
 #ifndef HD_ORACLE_H
 #define HD_ORACLE_H
@@ -21,7 +24,7 @@ enum { ORC_FLUX_CENTRAL = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_ROE = 2 };
 
 typedef struct orc_config {
     int nx, ny, nz;
-    int nbasis, iquad, nQ;       /* (nbasis,iquad) in {(4,2),(8,2),(10,3)}; nQ in {5,11} */
+    int nbasis, iquad, nQ;       /* (nbasis,iquad) in {(4,2),(8,2),(10,3)}; nQ in {5,11,13} */
     int ivis, ieos;              /* ivis in {0,1,2}; ieos 1 (ideal gas) or 2 (Murnaghan-Tait water) */
     int riemann;                 /* ORC_FLUX_* */
     int limiter;                 /* 1: positivity limiter on every stage input */
@@ -39,6 +42,7 @@ typedef struct orc_config {
     uint64_t seed;               /* noise key */
     uint32_t realization, pad2_;
     double eos_p1, eos_pbase;    /* ieos = 2: P = P_1*(rho^7.2 - 1) + P_base + P_ideal */
+    double coll_q, col_q;        /* nQ = 13: heat-flux relaxation rate and coll_q*conductivity */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;
@@ -88,6 +92,9 @@ void orc_noise_normals(uint64_t seed, uint32_t realization, uint64_t step, uint3
 /* stochastic stress (xx,yy,zz,xy,xz,yz) of point id in a stream at the current step index */
 int orc_noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta, double S[6]);
 
+/* threads of the cell and face loops when built with OpenMP (returns the
+   count in use; 1 without OpenMP) */
+int orc_set_threads(int n);
 const char *orc_last_error(void);
 int64_t orc_last_bad_cell(void);
 

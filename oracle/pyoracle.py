@@ -30,6 +30,7 @@ class OrcConfig(C.Structure):
         ("body_force", C.c_double * 3), ("eta_sd", C.c_double), ("zeta_sd", C.c_double),
         ("seed", C.c_uint64), ("realization", C.c_uint32), ("pad2_", C.c_uint32),
         ("eos_p1", C.c_double), ("eos_pbase", C.c_double),
+        ("coll_q", C.c_double), ("col_q", C.c_double),
     ]
 
 
@@ -37,7 +38,8 @@ def build(force: bool = False) -> str:
     """Compile the oracle with gcc (no FP contraction, like the Fortran build)."""
     src = os.path.join(HERE, "hd_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", LIB_PATH, src, "-lm"])
+        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-o", LIB_PATH, src,
+                               "-lm"])
     return LIB_PATH
 
 
@@ -75,6 +77,11 @@ def _configure(L):
     L.orc_sample.argtypes = [C.c_void_p, dp, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
     L.orc_last_error.restype = C.c_char_p
     L.orc_last_bad_cell.restype = C.c_int64
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = C.c_int
+    # one thread unless a caller asks for more (set_threads): the tests run
+    # several oracle processes side by side
+    L.orc_set_threads(1)
     return L
 
 
@@ -246,3 +253,8 @@ def noise_normals(seed, realization, step, stream, index):
     out = np.zeros(4)
     lib().orc_noise_normals(seed, realization, step, stream, index, _p(out))
     return out
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's cell and face loops (OpenMP); returns the count in use."""
+    return int(lib().orc_set_threads(int(n)))

@@ -422,3 +422,153 @@ def pin_nb10_third_order(Oracle):
         assert 2.8 < order < 3.3 and e[1] < 1.6e-4, (plane, e, order)
         p2 = [nb10_p2_rhs_error(Oracle, n, plane) for n in (16, 32)]
         assert p2[1] < 0.15 and p2[0] / p2[1] > 1.8, (plane, p2)
+
+
+# ---------------------------------------------------------- 13 moments ---
+# The 13-moment model has no counterpart in the reference: rh, mx, my, mz, en,
+# the traceless stress pxx, pyy, pxy, pxz, pyz (pzz = -pxx - pyy) and the heat
+# flux qx, qy, qz.  Its stress rows are those of the relaxation model; the heat
+# flux follows Maxwell-Cattaneo, d_t q_i + d_i(col_q T) = -coll_q q_i, T = p/rho,
+# and q enters the energy flux.
+T_PXX, T_PYY, T_PXY, T_PXZ, T_PYZ, T_QX = 5, 6, 7, 8, 9, 10
+STRESS13 = {(0, 1): T_PXY, (1, 0): T_PXY, (0, 2): T_PXZ, (2, 0): T_PXZ, (1, 2): T_PYZ, (2, 1): T_PYZ}
+
+
+def _cfg13(dims, lens, **kw):
+    base = dict(nx=dims[0], ny=dims[1], nz=dims[2], nbasis=4, iquad=2, nQ=13, ivis=1, coll=5.0, colvis=0.2,
+                coll_q=4.0, col_q=0.3, riemann="central", lx=lens[0], ly=lens[1], lz=lens[2], T_floor=1e-4,
+                limiter=False)
+    base.update(kw)
+    return HDConfig(**base)
+
+
+def wave13_errors(Oracle, axis, kind, pol=None, n=32, T=0.5, dt=0.002, A=1e-6, B=1e-6):
+    """As wave_errors, for the 13-moment model: shear waves, and the
+    longitudinal system (rho, u, p, pxx, pyy, q_axis) started from a velocity
+    wave plus a density wave at constant pressure, whose temperature
+    perturbation drives the heat flux."""
+    gam, rho0, p0 = 1.4, 1.0, 1.0
+    dims, lens = [1, 1, 1], [0.1, 0.1, 0.1]
+    dims[axis], lens[axis] = n, 1.0
+    cfg = _cfg13(dims, lens)
+    coll, colvis, collq, colq = cfg.coll, cfg.colvis, cfg.coll_q, cfg.col_q
+    o = Oracle(cfg.as_ints())
+    k, h = 2 * np.pi, 1.0 / n
+    if kind == "shear":
+        fields = [1 + pol, STRESS13[axis, pol]]
+        M = np.array([[0, -1j * k / rho0], [-1j * k * colvis, -coll]])
+        f0 = np.array([-1j * A, 0])
+    else:
+        # state [rho, u, p, pxx, pyy, q_axis]; the normal stress of the axis is
+        # pxx, pyy or pzz = -pxx - pyy
+        fields = [0, 1 + axis, 4, T_PXX, T_PYY, T_QX + axis]
+        M = np.zeros((6, 6), complex)
+        M[0, 1] = -1j * k * rho0
+        M[1, 2] = -1j * k / rho0
+        if axis < 2:
+            M[1, 3 + axis] = -1j * k / rho0
+        else:
+            M[1, 3] = M[1, 4] = 1j * k / rho0
+        M[2, 1] = -1j * k * gam * p0
+        M[2, 5] = -1j * k * (gam - 1)
+        for s, d in ((3, 0), (4, 1)):
+            M[s, 1] = -1j * k * colvis * (4.0 / 3.0 if d == axis else -2.0 / 3.0)
+            M[s, s] = -coll
+        M[5, 0] = 1j * k * colq * p0 / rho0 ** 2
+        M[5, 2] = -1j * k * colq / rho0
+        M[5, 5] = -collq
+        f0 = np.zeros(6, complex)
+        f0[1] = -1j * A
+        f0[0] = -B          # rho = B cos(kx): a temperature wave at constant pressure
+    fT = expm(M * T) @ f0
+    q = np.zeros(cfg.shape)
+    q[0, 0] = rho0
+    q[0, 4] = p0 / (gam - 1)
+    for f, fh in zip(fields, f0):
+        if fh != 0:
+            _add_sine(q, cfg, f, axis, k, np.angle(fh) + np.pi / 2, abs(fh))
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    for _ in range(int(round(T / dt))):
+        o.update(q, q1, q2, dt)
+    xc = ics.cell_centres(cfg)[axis]
+    err = 0.0
+    for f, fe in zip(fields, fT):
+        g = q[0, f].reshape(-1) - {0: rho0, 4: p0 / (gam - 1)}.get(f, 0.0)
+        fnum = (2.0 / n) * np.sum(g * np.exp(-1j * k * xc)) / _sinc(k * h / 2)
+        if f == 4:
+            fnum *= gam - 1
+        err = max(err, abs(fnum - fe) / A)
+    return err, np.abs(fT).min() / A
+
+
+def pin_13m_waves(Oracle):
+    for axis in range(3):
+        for pol in [d for d in range(3) if d != axis]:
+            e, smallest = wave13_errors(Oracle, axis, "shear", pol)
+            assert e < 0.01 and smallest > 0.1, ("shear", axis, pol, e)
+        e, smallest = wave13_errors(Oracle, axis, "long")
+        assert e < 0.01 and smallest > 0.02, ("long", axis, e, smallest)
+
+
+def _state13(cfg, seed):
+    rng = np.random.default_rng(seed)
+    q = ics.uniform(cfg, rho=1.0, T=1.0, v=(0.3, -0.2, 0.1))
+    q[1:, :5] = 0.02 * rng.standard_normal(q[1:, :5].shape)
+    q[0, :5] *= 1.0 + 0.03 * rng.standard_normal(q[0, :5].shape)
+    q[:, 5:] = 0.01 * rng.standard_normal(q[:, 5:].shape)
+    return q
+
+
+def pin_13m_invariants(Oracle):
+    for nb, qd in ((4, 2), (10, 3)):
+        for rm in ("central", "hllc", "roe"):
+            cfg = _cfg13((5, 4, 3), (1.0, 0.8, 0.6), nbasis=nb, iquad=qd, riemann=rm)
+            o = Oracle(cfg.as_ints())
+            # free stream, with uniform stress and heat flux: no flux divergence
+            u = ics.uniform(cfg, rho=1.3, T=0.7, v=(0.4, -0.3, 0.2))
+            u[0, 5:] = np.array([0.01, -0.02, 0.005, 0.003, -0.004, 0.02, -0.01, 0.007])[:, None, None, None]
+            assert np.abs(o.rhs(u)).max() < 1e-13, (nb, rm)
+            # periodic totals of every conserved field
+            r = o.rhs(_state13(cfg, nb))
+            tot = r[0].reshape(cfg.nQ, -1).sum(axis=1)
+            assert np.abs(tot[:5]).max() < 1e-11 * np.abs(r[0]).max(), (nb, rm, tot)
+    # relaxation at rest: each stage divides the stress by 1 + dt coll and the
+    # heat flux by 1 + dt coll_q (SSP-RK2 weights), with no other change
+    cfg = _cfg13((3, 3, 3), (1.0, 1.0, 1.0), colvis=0.0, col_q=0.0, coll=7.0, coll_q=3.0)
+    o = Oracle(cfg.as_ints())
+    q = ics.uniform(cfg)
+    q[0, T_PXY], q[0, T_QX + 1] = 0.3, -0.2
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    dt = 0.01
+    o.update(q, q1, q2, dt)
+    fp, fq = 1 / (1 + dt * 7.0), 1 / (1 + dt * 3.0)
+    np.testing.assert_allclose(q[0, T_PXY], 0.3 * 0.5 * (1 + fp * fp), rtol=1e-14)
+    np.testing.assert_allclose(q[0, T_QX + 1], -0.2 * 0.5 * (1 + fq * fq), rtol=1e-14)
+    np.testing.assert_allclose(q[0, :5], ics.uniform(cfg)[0, :5], rtol=1e-15)
+
+
+def _rotate13(a, nb):
+    """(x,y,z) -> (y,z,x) for a 13-moment state: new(x,y,z) = old(z,x,y)."""
+    perm = [0, 3, 1, 2] if nb == 4 else [0, 3, 1, 2, 6, 4, 5, 9, 7, 8]
+    b = a[perm].copy()
+    o = b.copy()
+    o[:, [1, 2, 3]] = b[:, [3, 1, 2]]
+    pzz = -(b[:, T_PXX] + b[:, T_PYY])
+    o[:, T_PXX] = pzz                 # new xx = old zz
+    o[:, T_PYY] = b[:, T_PXX]         # new yy = old xx
+    o[:, T_PXY] = b[:, T_PXZ]         # new xy = old zx
+    o[:, T_PXZ] = b[:, T_PYZ]         # new xz = old zy
+    o[:, T_PYZ] = b[:, T_PXY]         # new yz = old xy
+    o[:, [10, 11, 12]] = b[:, [12, 10, 11]]
+    return np.ascontiguousarray(np.transpose(o, (0, 1, 3, 4, 2)))
+
+
+def pin_13m_rotation(Oracle):
+    for nb, qd in ((4, 2), (10, 3)):
+        for rm in ("central", "roe"):
+            cfg = _cfg13((4, 4, 4), (1.0, 1.0, 1.0), nbasis=nb, iquad=qd, riemann=rm, ivis=0)
+            o = Oracle(cfg.as_ints())
+            u = _state13(cfg, 7)
+            r1 = o.rhs(u)
+            r2 = o.rhs(_rotate13(u, nb))
+            np.testing.assert_allclose(r2, _rotate13(r1, nb), rtol=0, atol=1e-12 * np.abs(r1).max())

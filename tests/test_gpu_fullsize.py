@@ -107,3 +107,98 @@ def test_vortex_context_holds_no_face_flux_arrays(torch_mod):
     s2 = Solver(cfg.replace(nx=8, ny=8, nz=8, ieos=2))
     face_flux_bytes = 8 * cfg.iquad ** 2 * cfg.nQ * 3 * (9 * 8 * 8)
     assert s2.device_bytes > face_flux_bytes
+
+
+def test_sod_256_to_t02_parity_and_exact_solution(torch_mod, oracle_mod):
+    """The 1D Sod tube at its benchmark size (256 cells, HLLC, limiter,
+    SSP-RK2, CFL 0.3, outflow) run with the reference's step semantics to
+    t = 0.2 on the GPU: the same state as the oracle (1e-12) and the same
+    time, and an L1 density error against the exact Riemann solution that a
+    second-order scheme reaches at this resolution."""
+    from hermeshd_amd.config import sod_1d
+    from hermeshd_amd.solver import Solver
+    from tests import refimpl
+    cfg = sod_1d(256)
+    q = ics.sod(cfg)
+    qd = torch_mod.from_numpy(q.copy()).cuda()
+    q1d = torch_mod.zeros_like(qd)
+    s = Solver(cfg)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    t, dt = np.array(0.0), np.array(0.0)
+    n = 0
+    while t < 0.2:
+        o.step(q, q1, q2, t, dt)
+        s.step(qd, q1d)
+        n += 1
+    s.check()
+    got = qd.cpu().numpy()
+    print(f"[sod256] {n} steps to t = {float(t):.6f}", flush=True)
+    assert abs(float(s.t.item()) - float(t)) <= 1e-12 * float(t)
+    assert field_rel_err(got, q) < 1e-12
+    xc, _, _ = ics.cell_centres(cfg)
+    rho_ex, _, _ = refimpl.exact_riemann(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, xc / float(t))
+    assert np.abs(got[0, 0, 0, 0] - rho_ex).mean() < 0.012
+
+
+def test_vortex_returns_after_one_period(torch_mod):
+    """The 128x128x32 isentropic vortex (nbasis 10, HLLC, limiter, SSP-RK3)
+    moves with the background flow (1, 1) across the 10 x 10 periodic box and
+    is back where it started at t = 10: the density of the cell averages
+    matches the initial field, and the density minimum sits in the same
+    cell."""
+    from hermeshd_amd.solver import Solver
+    cfg = vortex_3d()
+    q0 = ics.isentropic_vortex(cfg)
+    qd = torch_mod.from_numpy(q0.copy()).cuda()
+    q1d = torch_mod.zeros_like(qd)
+    s = Solver(cfg)
+    dt_cfl = float(s.min_dt(qd))
+    nsteps = int(np.ceil(10.0 / dt_cfl))
+    dt = torch_mod.full((), 10.0 / nsteps, dtype=torch_mod.float64, device=qd.device)
+    print(f"[vortex_period] {nsteps} steps of dt {10.0 / nsteps:.4e}", flush=True)
+    for _ in range(nsteps):
+        s.update(qd, q1d, None, dt)
+    s.check()
+    rho = qd[0, 0].cpu().numpy()
+    rho0 = q0[0, 0]
+    linf = np.abs(rho - rho0).max()
+    l1 = np.abs(rho - rho0).mean()
+    dip = 1.0 - rho0.min()
+    print(f"[vortex_period] density error max {linf:.3e} mean {l1:.3e} (vortex dip {dip:.3f})", flush=True)
+    assert np.unravel_index(np.argmin(rho), rho.shape) == np.unravel_index(np.argmin(rho0), rho0.shape)
+    assert linf < 5e-3 * dip and l1 < 1e-5
+
+
+@pytest.mark.parametrize("riemann,h", [("central", 1e-6), ("hllc", 1e-8)])
+def test_channel_gradient_full_size_vs_finite_differences(torch_mod, riemann, h):
+    """The adjoint workload at its benchmark size: the 64x128 relaxation
+    channel (nQ 11, ivis 1, no-slip walls, body force), 400 SSP-RK2 steps of
+    dt 0.05 with checkpoints every 20.  d J / d u0 and d J / d coll of the
+    wall-shear objective along a random direction against central
+    differences (1e-6)."""
+    from hermeshd_amd.adjoint import AdjointRunner, channel_state, run_with_grad, wall_shear
+    from hermeshd_amd.config import poiseuille_2d
+    cfg = poiseuille_2d().replace(riemann=riemann)
+    dev = torch_mod.device("cuda")
+    runner = AdjointRunner(cfg, dt=0.05, nsteps=400, checkpoint_every=20)
+    u0 = torch_mod.from_numpy(0.1 * ics.gaussian_field((1, cfg.ny, cfg.nx), 8.0, 1234)).to(dev)
+    coll0 = float(cfg.coll)
+
+    def J(u, c):
+        return wall_shear(run_with_grad(runner, channel_state(cfg, u), torch_mod.tensor(c, dtype=torch_mod.float64,
+                                                                                         device=dev)))
+
+    u = u0.clone().requires_grad_(True)
+    c = torch_mod.tensor(coll0, dtype=torch_mod.float64, device=dev, requires_grad=True)
+    gu, gc = torch_mod.autograd.grad(J(u, c), (u, c))
+    du = torch_mod.from_numpy(ics.gaussian_field((1, cfg.ny, cfg.nx), 4.0, 99)).to(dev)
+    with torch_mod.no_grad():
+        fd_u = (J(u0 + h * du, coll0) - J(u0 - h * du, coll0)) / (2 * h)
+        hc = h * coll0
+        fd_c = (J(u0, coll0 + hc) - J(u0, coll0 - hc)) / (2 * hc)
+    adj_u = float((gu * du).sum())
+    print(f"[channel_grad] {riemann} u: adjoint {adj_u:.12e} fd {float(fd_u):.12e}; coll: adjoint {float(gc):.12e} "
+          f"fd {float(fd_c):.12e}", flush=True)
+    assert abs(adj_u - float(fd_u)) <= 1e-6 * abs(float(fd_u))
+    assert abs(float(gc) - float(fd_c)) <= 1e-6 * abs(float(fd_c))

@@ -12,11 +12,13 @@ import numpy as np
 import pytest
 
 from hermeshd_amd import ics
-from hermeshd_amd.config import HDConfig, poiseuille_2d, sod_1d, vortex_3d
+from hermeshd_amd.config import HDConfig, poiseuille_2d, sod_1d, taylor_green_3d, vortex_3d
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 GROUPS = [[0], [1, 2, 3], [4], [5, 6, 7, 8, 9, 10]]
+# 13-moment model: traceless stress and heat flux
+GROUPS13 = [[0], [1, 2, 3], [4], [5, 6, 7, 8, 9], [10, 11, 12]]
 
 
 def field_rel_err(a, b):
@@ -25,7 +27,7 @@ def field_rel_err(a, b):
         return abs(a - b) / max(abs(b), 1e-300)
     nf = b.shape[1]
     err = 0.0
-    for grp in GROUPS:
+    for grp in (GROUPS13 if nf == 13 else GROUPS):
         grp = [f for f in grp if f < nf]
         if not grp:
             continue
@@ -40,7 +42,7 @@ def smooth_state(cfg, seed=0):
     q[1:, :5] = 0.02 * rng.standard_normal(q[1:, :5].shape)
     q[0, :5] *= 1.0 + 0.03 * rng.standard_normal(q[0, :5].shape)
     q[0, 4] += 0.5
-    if cfg.nQ == 11:
+    if cfg.nQ >= 11:
         q[:, 5:] = 0.01 * rng.standard_normal(q[:, 5:].shape)
     if cfg.ivis == 2:
         # the stress fields are second moments E_ij = rho u_i u_j + P_ij around
@@ -87,6 +89,18 @@ CASES = {
                                          colvis=0.3, ieos=2, riemann="central",
                                          bc=("wall", "wall", "noslip", "noslip", "periodic", "periodic"),
                                          lx=1.0, ly=0.8, lz=0.8, T_floor=1e-4, eos_p1=2.0, eos_pbase=1.0),
+    # 13-moment model (traceless stress + heat flux): table-driven kernels
+    "tm13_nb4_roe_periodic": HDConfig(nx=6, ny=5, nz=4, nbasis=4, iquad=2, nQ=13, ivis=1, coll=20.0, colvis=0.5,
+                                      coll_q=10.0, col_q=0.8, riemann="roe", lx=1.2, ly=1.0, lz=0.8, T_floor=1e-4),
+    "tm13_nb10_roe_walls": HDConfig(nx=5, ny=4, nz=6, nbasis=10, iquad=3, nQ=13, ivis=1, coll=30.0, colvis=0.4,
+                                    coll_q=12.0, col_q=0.6, riemann="roe",
+                                    bc=("periodic", "periodic", "noslip", "noslip", "wall", "wall"),
+                                    lx=1.0, ly=0.8, lz=1.2, T_floor=1e-4, integrator="shu-osher",
+                                    body_force=(1e-2, 0, 0)),
+    "tm13_nb10_hllc_ivis0_outflow": HDConfig(nx=4, ny=5, nz=4, nbasis=10, iquad=3, nQ=13, ivis=0, coll=3.0,
+                                             colvis=0.2, coll_q=2.0, col_q=0.3, riemann="hllc",
+                                             bc=("outflow",) * 6, lx=0.8, ly=1.0, lz=0.8, T_floor=1e-4),
+    "tgv_nb10_roe_8": taylor_green_3d(8),
 }
 
 
@@ -150,7 +164,9 @@ def test_min_dt_parity(torch_mod, oracle_mod, name):
                                          ("relax_nb10_roe", 5), ("tenmom_nb4_hllc_periodic", 6),
                                          ("tenmom_nb10_roe_walls", 3), ("tenmom_nb8_central_outflow", 4),
                                          ("water_nb4_hllc_periodic", 5), ("water_nb10_central_walls", 3),
-                                         ("vortex_nb10_rows64", 4), ("relax_nq11_rows64", 6)])
+                                         ("vortex_nb10_rows64", 4), ("relax_nq11_rows64", 6),
+                                         ("tm13_nb4_roe_periodic", 8), ("tm13_nb10_roe_walls", 4),
+                                         ("tm13_nb10_hllc_ivis0_outflow", 3), ("tgv_nb10_roe_8", 6)])
 def test_step_parity(torch_mod, oracle_mod, name, nsteps):
     """Reference step semantics (dt from get_min_dt, RK update, t += dt) for N steps."""
     from hermeshd_amd.solver import Solver
@@ -163,6 +179,8 @@ def test_step_parity(torch_mod, oracle_mod, name, nsteps):
         q = ics.isentropic_vortex(cfg, jitter=1e-6)
     elif name.startswith("relax_nq11"):
         q = ics.channel_flow(cfg, corr_cells=2.0)
+    elif name.startswith("tgv"):
+        q = ics.taylor_green(cfg)
     else:
         q = smooth_state(cfg, seed=3)
     qd = _dev(q)

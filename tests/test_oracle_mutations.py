@@ -58,6 +58,18 @@ MUTATIONS = [
      "u1 = u1 + w[p] * 3. * c->bint[p][KX] * fx[p * nQ + f];",
      "u1 = u1 + w[p] * 1.5 * c->bint[p][KX] * fx[p * nQ + f];",
      ["pin_nb10_third_order"]),
+    ("13-moment heat-flux row in the y flux",
+     "F[T_QX] = 0; F[T_QY] = ct; F[T_QZ] = 0;",
+     "F[T_QX] = ct; F[T_QY] = 0; F[T_QZ] = 0;",
+     ["pin_13m_rotation", "pin_13m_waves"]),
+    ("13-moment heat flux left out of the x energy flux",
+     "F[EN] = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz + Q[T_QX];",
+     "F[EN] = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz;",
+     ["pin_13m_rotation", "pin_13m_waves"]),
+    ("13-moment pzz with the wrong sign",
+     "T_zz = -(T_xx + T_yy);",
+     "T_zz = (T_xx + T_yy);",
+     ["pin_13m_rotation", "pin_13m_waves"]),
     ("P2 normalisation halved",
      "c->cb[KXX] = c->cb[KYY] = c->cb[KZZ] = 5.;",
      "c->cb[KXX] = c->cb[KYY] = c->cb[KZZ] = 2.5;",

@@ -896,3 +896,21 @@ def test_boundary_branches_closed_form(oracle_mod):
 def test_quadratic_basis_third_order(oracle_mod):
     from tests import pins
     pins.pin_nb10_third_order(oracle_mod.Oracle)
+
+
+# ---------------------------------------------------------------------------
+# 13-moment model (no counterpart in the reference; tests/pins.py)
+# ---------------------------------------------------------------------------
+def test_13m_free_stream_conservation_relaxation(oracle_mod):
+    from tests import pins
+    pins.pin_13m_invariants(oracle_mod.Oracle)
+
+
+def test_13m_rotation_symmetry(oracle_mod):
+    from tests import pins
+    pins.pin_13m_rotation(oracle_mod.Oracle)
+
+
+def test_13m_waves_follow_dispersion_relation(oracle_mod):
+    from tests import pins
+    pins.pin_13m_waves(oracle_mod.Oracle)

@@ -28,6 +28,17 @@ CFGS = {
                                            colvis=0.5, riemann="hllc",
                                            bc=("inflow", "outflow", "noslip", "noslip", "periodic", "periodic"),
                                            lx=1.0, ly=1.2, lz=0.2, T_floor=1e-4),
+    # Roe (legacy solver) and the 13-moment model
+    "nb4_nq11_roe_walls_ivis1": HDConfig(nx=5, ny=4, nz=3, nbasis=4, iquad=2, nQ=11, ivis=1, coll=15.0, colvis=0.4,
+                                         riemann="roe", bc=("noslip", "noslip", "periodic", "periodic", "wall",
+                                                            "wall"), lx=1.0, ly=0.8, lz=0.6, T_floor=1e-4),
+    "nb4_nq13_roe_periodic_ivis1": HDConfig(nx=4, ny=5, nz=3, nbasis=4, iquad=2, nQ=13, ivis=1, coll=20.0,
+                                            colvis=0.5, coll_q=10.0, col_q=0.8, riemann="roe", lx=0.8, ly=1.0,
+                                            lz=0.6, T_floor=1e-4),
+    "nb10_nq13_roe_walls_ivis0": HDConfig(nx=3, ny=4, nz=3, nbasis=10, iquad=3, nQ=13, ivis=0, coll=3.0,
+                                          colvis=0.2, coll_q=2.0, col_q=0.3, riemann="roe",
+                                          bc=("periodic", "periodic", "noslip", "noslip", "wall", "wall"),
+                                          lx=0.9, ly=1.2, lz=0.9, T_floor=1e-4),
 }
 
 
@@ -70,8 +81,9 @@ def test_transcription_limiter_matches_oracle(oracle_mod, name):
     assert field_rel_err(lim, q) < 1e-13
 
 
-def test_transcription_euler_step_matches_oracle(oracle_mod):
-    cfg = CFGS["nb4_nq11_hllc_noslip_ivis1"]
+@pytest.mark.parametrize("name", ["nb4_nq11_hllc_noslip_ivis1", "nb4_nq13_roe_periodic_ivis1"])
+def test_transcription_euler_step_matches_oracle(oracle_mod, name):
+    cfg = CFGS[name]
     o = oracle_mod.Oracle(cfg.as_ints())
     q = smooth_state(cfg, seed=9)
     out = np.zeros_like(q)

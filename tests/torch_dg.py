@@ -22,6 +22,8 @@ import torch
 
 RH, MX, MY, MZ, EN = 0, 1, 2, 3, 4
 PXX, PYY, PZZ, PXY, PXZ, PYZ = 5, 6, 7, 8, 9, 10
+# 13-moment model: traceless stress without pzz (= -pxx - pyy), heat flux
+T_PXX, T_PYY, T_PXY, T_PXZ, T_PYZ, T_QX = 5, 6, 7, 8, 9, 10
 BC = {"periodic": 0, "outflow": 1, "wall": 2, "noslip": 3, "inflow": 4}
 
 
@@ -73,6 +75,7 @@ class DG:
         self.P_floor = cfg.T_floor * cfg.rh_floor
         self.rh_floor = cfg.rh_floor
         self.colvis = cfg.colvis
+        self.colq = getattr(cfg, "col_q", 0.0)
         self.bc = [BC[b] for b in cfg.bc]
 
     # ----------------------------------------------------------- points ----
@@ -83,8 +86,34 @@ class DG:
         P = torch.where(P < self.P_floor, torch.full_like(P, self.P_floor), P)
         return ri, u, P
 
+    def flux13(self, Q, d):
+        """13-moment flux in direction d: the relaxation model's stress rows for
+        the five stored components (pzz = -pxx - pyy) and the Maxwell-Cattaneo
+        heat flux (flux col_q*T of q_d; q_d in the energy flux)."""
+        ri, u, P = self.prim(Q)
+        un = u[d]
+        tzz = -(Q[T_PXX] + Q[T_PYY])
+        rows = [(Q[T_PXX], Q[T_PXY], Q[T_PXZ]), (Q[T_PXY], Q[T_PYY], Q[T_PYZ]), (Q[T_PXZ], Q[T_PYZ], tzz)][d]
+        F = [None] * 13
+        F[RH] = Q[MX + d]
+        for a in range(3):
+            F[MX + a] = Q[MX + a] * un + rows[a] + (P if a == d else 0.0)
+        F[EN] = (Q[EN] + P) * un + rows[0] * u[0] + rows[1] * u[1] + rows[2] * u[2] + Q[T_QX + d]
+        c4, c2, cv = 4.0 / 3.0 * self.colvis, 2.0 / 3.0 * self.colvis, self.colvis
+        z = torch.zeros_like(un)
+        F[T_PXX] = c4 * un if d == 0 else -c2 * un
+        F[T_PYY] = c4 * un if d == 1 else -c2 * un
+        F[T_PXY] = cv * u[1] if d == 0 else (cv * u[0] if d == 1 else z)
+        F[T_PXZ] = cv * u[2] if d == 0 else (cv * u[0] if d == 2 else z)
+        F[T_PYZ] = cv * u[2] if d == 1 else (cv * u[1] if d == 2 else z)
+        for a in range(3):
+            F[T_QX + a] = self.colq * P * ri if a == d else z
+        return torch.stack(F)
+
     def flux(self, Q, d):
         """Q: (nQ, ...) -> F (nQ, ...) in direction d (flux_calc_pnts_r, ivis 0/1)."""
+        if self.nQ == 13:
+            return self.flux13(Q, d)
         ri, u, P = self.prim(Q)
         un = u[d]
         F = [None] * self.nQ
@@ -113,8 +142,10 @@ class DG:
         out = 0.5 * (FL + FR)
         if self.cfg.riemann == "central":
             return out
+        if self.cfg.riemann == "roe":
+            return torch.cat([self.roe(QL, QR, FL, FR, d), out[5:]], dim=0)
         if self.cfg.riemann != "hllc":
-            raise NotImplementedError("the transcription has central and HLLC fluxes")
+            raise NotImplementedError("the transcription has central, HLLC and Roe fluxes")
         g, gm1 = self.gamma, self.gm1
         IP, I1, I2 = MX + d, MX + (d + 1) % 3, MX + (d + 2) % 3
         rL, rR = QL[RH], QR[RH]
@@ -158,6 +189,58 @@ class DG:
             v = torch.where(sel == 3, FR[f], v)
             rows.append(v)
         return torch.cat([torch.stack(rows), out[5:]], dim=0)
+
+    def roe(self, QL, QR, FL, FR, d):
+        """Roe flux of rh..en (legacy Eulderink-Mellema form with half the
+        thermal enthalpy, src/ces/10_moment/dg_3D_hydro_lqc_SEAN.f90:1614-1789);
+        local Lax-Friedrichs where the Roe sound speed squared is not positive."""
+        gm1, cp = self.gm1, self.gamma / self.gm1
+        IP, I1, I2 = MX + d, MX + (d + 1) % 3, MX + (d + 2) % 3
+        sk, sw = [], []
+        for Q in (QL, QR):
+            a, b, c = Q[IP] / Q[RH], Q[I1] / Q[RH], Q[I2] / Q[RH]
+            vsq = a * a + b * b + c * c
+            Pri = gm1 * (Q[EN] - 0.5 * Q[RH] * vsq)
+            k = torch.sqrt(Q[RH])
+            sk.append(k)
+            sw.append([k * a, k * b, k * c, 0.5 * k * (vsq + cp * Pri / Q[RH])])
+        Qri = 1.0 / QL[RH]
+        delrho = QR[RH] * Qri - 1.0
+        dele = (QR[EN] - QL[EN]) * Qri
+        dm1, dm2, dm3 = ((QR[f] - QL[f]) * Qri for f in (IP, I1, I2))
+        dsk = 0.5 * (sk[1] + sk[0])
+        vc0, vc1, vc2, hc = (0.5 * (sw[1][j] + sw[0][j]) / dsk for j in range(4))
+        vcsq = vc0 * vc0 + vc1 * vc1 + vc2 * vc2
+        asq = gm1 * (hc - 0.5 * vcsq)
+        ok = asq > 0
+        asq_s = torch.where(ok, asq, torch.ones_like(asq))
+        sc = torch.sqrt(asq_s)
+        l0, l1, l2 = (vc0 - sc).abs(), (vc0 + sc).abs(), vc0.abs()
+        ea2i = gm1 / asq_s
+        vcdel = vc0 * dm1 + vc1 * dm2 + vc2 * dm3 - dele
+        a9e = 0.5 * ea2i * (0.5 * vcsq * delrho - vcdel)
+        sk9 = 0.5 * (dm1 - vc0 * delrho) / sc
+        a0, a1 = a9e - sk9, a9e + sk9
+        a2 = ea2i * ((hc - vcsq) * delrho + vcdel)
+        a3, a4 = dm2 - vc1 * delrho, dm3 - vc2 * delrho
+        t0, t1, t2, t3, t4 = a0 * l0, a1 * l1, a2 * l2, a3 * l2, a4 * l2
+        dd = {RH: t0 + t1 + t2,
+              EN: t0 * (hc - sc * vc0) + t1 * (hc + sc * vc0) + t2 * (0.5 * vcsq) + t3 * vc1 + t4 * vc2,
+              IP: t0 * (vc0 - sc) + t1 * (vc0 + sc) + t2 * vc0,
+              I1: (t0 + t1 + t2) * vc1 + t3, I2: (t0 + t1 + t2) * vc2 + t4}
+
+        def cfcal(Q):
+            ri = 1.0 / Q[RH]
+            v = [Q[MX + a] * ri for a in range(3)]
+            P = gm1 * (Q[EN] - 0.5 * Q[RH] * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+            return v[d].abs() + torch.sqrt(self.gamma * P * ri)
+        cw = torch.fmax(cfcal(QL), cfcal(QR))
+        rows = []
+        for f in range(5):
+            roe = 0.5 * (FL[f] + FR[f] - dd[f] * QL[RH])
+            llf = 0.5 * (FL[f] + FR[f]) - 0.5 * cw * (QR[f] - QL[f])
+            rows.append(torch.where(ok, roe, llf))
+        return torch.stack(rows)
 
     # ------------------------------------------------------------- faces ----
     def _trace(self, q, d, side):
@@ -240,8 +323,10 @@ class DG:
         # sources: explicit relaxation (ivis 0), body force
         src = torch.zeros_like(q)
         c = cfg.coll if coll is None else coll
-        if self.nQ >= 11 and cfg.ivis == 0 and cfg.coll != 0.0:
+        if self.nQ == 11 and cfg.ivis == 0 and cfg.coll != 0.0:
             src = torch.cat([src[:, :PXX], -c * q[:, PXX:]], dim=1)
+        if self.nQ == 13 and cfg.ivis == 0:
+            src = torch.cat([src[:, :PXX], -c * q[:, PXX:T_QX], -cfg.coll_q * q[:, T_QX:]], dim=1)
         bf = cfg.body_force
         if any(b != 0.0 for b in bf):
             rows = [src[:, f] for f in range(self.nQ)]
@@ -297,5 +382,9 @@ class DG:
         if self.cfg.ivis == 1 and self.nQ >= 11:
             c = self.cfg.coll if coll is None else coll
             faci = 1.0 / (1.0 + dt * c)
-            v = torch.cat([v[:, :PXX], faci * v[:, PXX:]], dim=1)
+            if self.nQ == 13:
+                faciq = 1.0 / (1.0 + dt * self.cfg.coll_q)
+                v = torch.cat([v[:, :PXX], faci * v[:, PXX:T_QX], faciq * v[:, T_QX:]], dim=1)
+            else:
+                v = torch.cat([v[:, :PXX], faci * v[:, PXX:]], dim=1)
         return v
