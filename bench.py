@@ -13,8 +13,8 @@ Prints one JSON line (rank 0, N = 1):
              against 8 TB/s; traffic = HBM bytes per stage-kernel dispatch
              from rocprofv3 FETCH_SIZE + WRITE_SIZE (two child runs, each
              scaled by the counter on a copy of known size)
-  cpu_baseline  the C oracle (one thread) on a bounded sample of the same
-             workload on this host.
+  cpu_baseline  the C oracle on the host cores (OpenMP, at most 16
+             threads), one step of the same workload.
 """
 from __future__ import annotations
 
@@ -43,25 +43,27 @@ def stage_bytes(cfg, stages):
 
 
 def cpu_baseline(cfg, make_ic, stages):
-    """Times the oracle (one thread) on a bounded sample: one reference step
-    (get_min_dt + all RK stages) of the same workload on a z-slab of at most
-    16 cells (same per-cell work, periodic in z), ~10-30 s."""
+    """Times the C oracle on the host cores: one reference step (get_min_dt +
+    all RK stages) of the whole workload, with the oracle's cell and face
+    loops spread over min(16, host cores) OpenMP threads (16 is this GPU's
+    share of the host).  Results are bitwise those of one thread."""
     import numpy as np
     from oracle import pyoracle
     pyoracle.build()
-    nz = min(cfg.nz, 16)
-    sub = cfg.replace(nz=nz, lz=cfg.lz * nz / cfg.nz)
-    o = pyoracle.Oracle(sub.as_ints())
-    q = make_ic(sub)
+    cores = pyoracle.set_threads(min(16, os.cpu_count() or 1))
+    o = pyoracle.Oracle(cfg.as_ints())
+    q = make_ic(cfg)
     q1, q2 = np.zeros_like(q), np.zeros_like(q)
     t, dt = np.array(0.0), np.array(0.0)
     t0 = time.perf_counter()
     o.step(q, q1, q2, t, dt)
     el = time.perf_counter() - t0
-    dofs = sub.ncell * sub.nQ * sub.nbasis
-    return {"value": dofs * stages / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-            "sample": f"one reference step ({stages} RK stages + get_min_dt) of the same workload on a "
-                      f"{sub.nx}x{sub.ny}x{sub.nz} slab by the C oracle, 1 thread, {el:.1f} s"}
+    pyoracle.set_threads(1)
+    dofs = cfg.ncell * cfg.nQ * cfg.nbasis
+    return {"value": dofs * stages / el, "unit": "DOF-updates/s", "cores": cores, "kind": "port",
+            "sample": f"one reference step ({stages} RK stages + get_min_dt) of the whole "
+                      f"{cfg.nx}x{cfg.ny}x{cfg.nz} workload by the C oracle, {cores} OpenMP threads, "
+                      f"{el:.1f} s"}
 
 
 # ---------------------------------------------------------------------------
@@ -178,6 +180,65 @@ def adjoint_leg(nsteps=400, checkpoint_every=20):
             "J_wall_shear": J, "dJ_dvis": gv, "norm_dJ_du0": gun}
 
 
+def taylor_green_leg(n=192, steps=4, adj_n=48, adj_steps=20, checkpoint_every=5):
+    """13-moment Taylor-Green vortex (nbasis 10, nQ 13, Roe, limiter on,
+    SSP-RK3, Re 1600): the forward step on the n^3 grid (DOF-updates/s per
+    RK stage at the CFL time step), and the cost of the gradient of the
+    kinetic energy after adj_steps steps with respect to the initial velocity
+    field on adj_n^3 cells (forward run vs backward: recompute from
+    checkpoints + stage transposes)."""
+    import torch
+    from hermeshd_amd import ics
+    from hermeshd_amd.adjoint import AdjointRunner, kinetic_energy, run_with_grad, velocity_state
+    from hermeshd_amd.config import taylor_green_3d
+    from hermeshd_amd.solver import Solver
+    dev = torch.device("cuda:0")
+    cfg = taylor_green_3d(n)
+    s = Solver(cfg, device=dev)
+    q = torch.from_numpy(ics.taylor_green(cfg)).to(dev)
+    q1 = torch.zeros_like(q)
+    s.step(q, q1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(q, q1)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) / steps * 1e3
+    s.check()
+    dofs = cfg.ncell * cfg.nQ * cfg.nbasis
+    dt = float(s.dt)
+    del q, q1, s
+    torch.cuda.empty_cache()
+    # adjoint cost on a smaller grid (the stage records of a checkpoint segment
+    # hold nine states per step)
+    acfg = taylor_green_3d(adj_n)
+    qa = torch.from_numpy(ics.taylor_green(acfg)).to(dev)
+    vel0 = (qa[0, 1:4] / qa[0, 0]).clone()
+    adt = float(Solver(acfg, device=dev).min_dt(qa))
+    runner = AdjointRunner(acfg, dt=adt, nsteps=adj_steps, checkpoint_every=checkpoint_every)
+    coll = torch.tensor(float(acfg.coll), dtype=torch.float64, device=dev)
+
+    def once():
+        v = vel0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        a0 = time.perf_counter()
+        J = kinetic_energy(run_with_grad(runner, velocity_state(acfg, v), coll), acfg)
+        torch.cuda.synchronize()
+        a1 = time.perf_counter()
+        (g,) = torch.autograd.grad(J, v)
+        torch.cuda.synchronize()
+        a2 = time.perf_counter()
+        return a1 - a0, a2 - a1, float(J), float(g.norm())
+    once()
+    tf, tb, J, gn = once()
+    return {"workload": f"13-moment Taylor-Green vortex {n}^3 on [0,2pi]^3, nbasis 10, nQ 13, Roe, limiter, SSP-RK3, "
+                        f"Re 1600, Pr 0.71",
+            "ms_per_step": round(ms_step, 2), "dt": dt, "dof_updates_per_s": dofs * 3 / (ms_step * 1e-3),
+            "adjoint": {"grid": f"{adj_n}^3", "steps": adj_steps, "checkpoint_every": checkpoint_every,
+                        "forward_ms": round(tf * 1e3, 2), "backward_ms": round(tb * 1e3, 2),
+                        "ratio": round(tb / tf, 3), "KE": J, "norm_dKE_du0": gn}}
+
+
 def ensemble_leg(steps=20):
     """Batched ensemble (96x16^3 + 96x24^3 + 64x32^3 periodic boxes, nbasis 4,
     nQ 11, stochastic stress, HLLC, SSP-RK2, dt 0.1, graph-replayed steps,
@@ -243,6 +304,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-tgv", action="store_true", help="skip the 13-moment Taylor-Green leg")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -350,12 +412,16 @@ def main():
                       "stages_per_step": stages, "parallelism": "single GPU"},
            "roofline": roofline}
     if not args.no_cpu_baseline:
-        log("bench: timing the CPU baseline (oracle, 1 thread)...")
+        log("bench: timing the CPU baseline (oracle, OpenMP threads)...")
         out["cpu_baseline"] = cpu_baseline(cfg, make_ic, stages)
     if not args.no_adjoint:
         log("bench: adjoint leg (relaxation channel, 400 steps)...")
         out["adjoint"] = adjoint_leg()
         log(f"bench: adjoint {out['adjoint']}")
+    if not args.no_tgv:
+        log("bench: 13-moment Taylor-Green leg...")
+        out["taylor_green"] = taylor_green_leg()
+        log(f"bench: taylor_green {out['taylor_green']}")
     if not args.no_ensemble:
         log("bench: ensemble leg (256 realizations)...")
         out["ensemble"] = ensemble_leg()

@@ -3,7 +3,7 @@
 // Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11) with counter (index lo, index hi | stream<<24,
 // step lo, step hi) and key = seed mixed with the realization id; uniforms
-// (u + 0.5) 2^-32 and Box-Muller pairs.  A value depends only on (seed,
+// (u + 0.5) 2^-32 and Box-Muller pairs in single precision.  A value depends only on (seed,
 // realization, step, stream, index), never on the launch shape.
 //
 // Stress at one point (random_stresses_pnts_r, src/random.f90:106-145, with
@@ -39,24 +39,77 @@ __device__ __forceinline__ void philox4x32_10(uint32_t& c0, uint32_t& c1, uint32
     }
 }
 
+// Single-precision Box-Muller (the reference draws its Gaussians in single
+// precision, vsRngGaussian, src/random.f90:61,87).  The logarithm and the
+// sine/cosine are written out here operation by operation, with no
+// contraction into fused multiply-adds, from correctly rounded FP32
+// operations only (+, -, *, /, sqrt, conversions), so that any IEEE FP32
+// implementation of the same steps gives the same bits.
+//   log a = e ln2 + 2 atanh(s),  a = m 2^e, m in [sqrt(1/2), sqrt(2)),
+//           s = (m-1)/(m+1), atanh by its odd series to s^9 (|s| < 0.172)
+//   sin/cos(pi x): x = n/2 + r, |r| <= 1/4 (quadrant n), Taylor series of
+//           sin and cos of pi r to degree 9 and 10, then the quadrant's signs
+__device__ __forceinline__ float bm_logf(float a) {
+#pragma clang fp contract(off)
+    uint32_t bits = __float_as_uint(a);
+    int e = (int)((bits >> 23) & 0xffu) - 127;
+    float m = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 0x1.c71c72p-4f;       // 1/9
+    p = p * s2 + 0x1.24924ap-3f;    // 1/7
+    p = p * s2 + 0x1.99999ap-3f;    // 1/5
+    p = p * s2 + 0x1.555556p-2f;    // 1/3
+    p = p * s2 + 1.0f;
+    const float lm = 2.0f * s * p;
+    return (float)e * 0x1.62e430p-1f + lm;  // ln 2
+}
+
+__device__ __forceinline__ void bm_sincospi(float x, float* sv, float* cv) {
+#pragma clang fp contract(off)
+    const float nf = floorf(2.0f * x + 0.5f);
+    const int n = (int)nf;
+    const float r = x - nf * 0.5f;
+    const float z = r * 0x1.921fb6p+1f;  // pi
+    const float z2 = z * z;
+    float sp = 0x1.71de3ap-19f;          // 1/9!
+    sp = sp * z2 - 0x1.a01a02p-13f;      // 1/7!
+    sp = sp * z2 + 0x1.111112p-7f;       // 1/5!
+    sp = sp * z2 - 0x1.555556p-3f;       // 1/3!
+    sp = sp * z2 + 1.0f;
+    const float sn = z * sp;
+    float cp = -0x1.27e4fcp-22f;         // 1/10!
+    cp = cp * z2 + 0x1.a01a02p-16f;      // 1/8!
+    cp = cp * z2 - 0x1.6c16c2p-10f;      // 1/6!
+    cp = cp * z2 + 0x1.555556p-5f;       // 1/4!
+    cp = cp * z2 - 0.5f;
+    const float cs = cp * z2 + 1.0f;
+    const int qd = n & 3;
+    *sv = qd == 0 ? sn : (qd == 1 ? cs : (qd == 2 ? -sn : -cs));
+    *cv = qd == 0 ? cs : (qd == 1 ? -sn : (qd == 2 ? -cs : sn));
+}
+
 __device__ __forceinline__ void normals4(uint64_t seed, uint32_t realization, uint64_t step, uint32_t stream,
                                          uint64_t index, double* out) {
+#pragma clang fp contract(off)
     uint32_t c0 = (uint32_t)index, c1 = (uint32_t)(index >> 32) | (stream << 24);
     uint32_t c2 = (uint32_t)step, c3 = (uint32_t)(step >> 32);
     philox4x32_10(c0, c1, c2, c3, (uint32_t)seed ^ (realization * 0x9E3779B9u), (uint32_t)(seed >> 32) ^ realization);
-    const double two_m32 = 2.3283064365386963e-10;
     const uint32_t u[4] = {c0, c1, c2, c3};
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const double a = ((double)u[2 * h] + 0.5) * two_m32;
-        const double b = ((double)u[2 * h + 1] + 0.5) * two_m32;
-        const double rad = sqrt(-2.0 * log(a));
-        // cos/sin(2 pi b) through the pi-scaled forms (no large-argument
-        // reduction); equal to cos/sin(2 pi b) to an ulp or two
-        double sv, cv;
-        sincospi(2.0 * b, &sv, &cv);
-        out[2 * h] = rad * cv;
-        out[2 * h + 1] = rad * sv;
+        // uniforms in (0, 1]: (u + 1/2) 2^-32 in FP32
+        const float a = ((float)u[2 * h] + 0.5f) * 0x1p-32f;
+        const float b = ((float)u[2 * h + 1] + 0.5f) * 0x1p-32f;
+        const float rad = sqrtf(-2.0f * bm_logf(a));
+        float sv, cv;
+        bm_sincospi(2.0f * b, &sv, &cv);
+        out[2 * h] = (double)(rad * cv);
+        out[2 * h + 1] = (double)(rad * sv);
     }
 }
 

@@ -335,9 +335,63 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* Single-precision Box-Muller (the reference draws its Gaussians in single
+   precision, vsRngGaussian, src/random.f90:61,87), from correctly rounded
+   FP32 operations only, in a fixed order with no contraction (this file is
+   built with -ffp-contract=off), so that every IEEE FP32 implementation of
+   the same steps gives the same bits:
+     log a = e ln2 + 2 atanh(s), a = m 2^e, m in [sqrt(1/2), sqrt(2)),
+             s = (m-1)/(m+1), atanh by its odd series to s^9 (|s| < 0.172);
+     sin/cos(pi x): x = n/2 + r, |r| <= 1/4, Taylor series of sin and cos of
+             pi r to degree 9 and 10, then the signs of quadrant n. */
+static float bm_logf(float a) {
+    uint32_t bits;
+    memcpy(&bits, &a, 4);
+    int e = (int)((bits >> 23) & 0xffu) - 127;
+    bits = (bits & 0x007fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &bits, 4);
+    if (m > 0x1.6a09e6p+0f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 0x1.c71c72p-4f;
+    p = p * s2 + 0x1.24924ap-3f;
+    p = p * s2 + 0x1.99999ap-3f;
+    p = p * s2 + 0x1.555556p-2f;
+    p = p * s2 + 1.0f;
+    const float lm = 2.0f * s * p;
+    return (float)e * 0x1.62e430p-1f + lm;
+}
+
+static void bm_sincospi(float x, float *sv, float *cv) {
+    const float nf = floorf(2.0f * x + 0.5f);
+    const int n = (int)nf;
+    const float r = x - nf * 0.5f;
+    const float z = r * 0x1.921fb6p+1f;
+    const float z2 = z * z;
+    float sp = 0x1.71de3ap-19f;
+    sp = sp * z2 - 0x1.a01a02p-13f;
+    sp = sp * z2 + 0x1.111112p-7f;
+    sp = sp * z2 - 0x1.555556p-3f;
+    sp = sp * z2 + 1.0f;
+    const float sn = z * sp;
+    float cp = -0x1.27e4fcp-22f;
+    cp = cp * z2 + 0x1.a01a02p-16f;
+    cp = cp * z2 - 0x1.6c16c2p-10f;
+    cp = cp * z2 + 0x1.555556p-5f;
+    cp = cp * z2 - 0.5f;
+    const float cs = cp * z2 + 1.0f;
+    const int qd = n & 3;
+    *sv = qd == 0 ? sn : (qd == 1 ? cs : (qd == 2 ? -sn : -cs));
+    *cv = qd == 0 ? cs : (qd == 1 ? -sn : (qd == 2 ? -cs : sn));
+}
+
 /* Four N(0,1) numbers for one counter value.  Counter layout:
    ctr = (index lo, index hi | stream<<24, step lo, step hi), key = seed ^ (realization mix).
-   Uniforms in (0,1): (u + 0.5) * 2^-32; Box-Muller on pairs. */
+   Uniforms in (0,1]: (u + 0.5) * 2^-32 in FP32; Box-Muller on pairs. */
 void orc_noise_normals(uint64_t seed, uint32_t realization, uint64_t step, uint32_t stream,
                        uint64_t index, double out[4]) {
     uint32_t ctr[4] = {(uint32_t)index, (uint32_t)(index >> 32) | (stream << 24), (uint32_t)step,
@@ -345,14 +399,14 @@ void orc_noise_normals(uint64_t seed, uint32_t realization, uint64_t step, uint3
     uint32_t key[2] = {(uint32_t)seed ^ (realization * 0x9E3779B9u), (uint32_t)(seed >> 32) ^ realization};
     uint32_t u[4];
     orc_philox4x32_10(ctr, key, u);
-    const double two_m32 = 2.3283064365386963e-10;
     for (int h = 0; h < 2; h++) {
-        double a = ((double)u[2 * h] + 0.5) * two_m32;
-        double b = ((double)u[2 * h + 1] + 0.5) * two_m32;
-        double rad = sqrt(-2.0 * log(a));
-        double ang = 6.283185307179586 * b;
-        out[2 * h] = rad * cos(ang);
-        out[2 * h + 1] = rad * sin(ang);
+        const float a = ((float)u[2 * h] + 0.5f) * 0x1p-32f;
+        const float b = ((float)u[2 * h + 1] + 0.5f) * 0x1p-32f;
+        const float rad = sqrtf(-2.0f * bm_logf(a));
+        float sv, cv;
+        bm_sincospi(2.0f * b, &sv, &cv);
+        out[2 * h] = (double)(rad * cv);
+        out[2 * h + 1] = (double)(rad * sv);
     }
 }
 

@@ -359,6 +359,23 @@ def test_philox_known_answers(oracle_mod):
         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
+def test_noise_normals_are_box_muller_of_the_philox_words(oracle_mod):
+    """The single-precision Box-Muller (its own log and sin/cos polynomials)
+    against double-precision Box-Muller of the same Philox-4x32-10 words:
+    the FP32 uniforms and transforms stay within 5e-6 of it."""
+    err = 0.0
+    seed, real, step, stream = 1234, 3, 7, 1
+    key = [(seed ^ (real * 0x9E3779B9)) & 0xFFFFFFFF, (seed >> 32) ^ real]
+    for i in range(4000):
+        g = oracle_mod.noise_normals(seed, real, step, stream, i)
+        u = oracle_mod.philox4x32_10([i & 0xFFFFFFFF, (i >> 32) | (stream << 24), step, 0], key)
+        for h in range(2):
+            a, b = (u[2 * h] + 0.5) * 2.0 ** -32, (u[2 * h + 1] + 0.5) * 2.0 ** -32
+            r = math.sqrt(-2.0 * math.log(a))
+            err = max(err, abs(g[2 * h] - r * math.cos(2 * math.pi * b)), abs(g[2 * h + 1] - r * math.sin(2 * math.pi * b)))
+    assert err < 5e-6, err
+
+
 def test_noise_normals_moments(oracle_mod):
     vals = np.concatenate([oracle_mod.noise_normals(1234, 3, 7, 1, i) for i in range(20000)])
     assert abs(vals.mean()) < 0.02 and abs(vals.std() - 1) < 0.02
