@@ -83,6 +83,11 @@ struct orc_ctx {
     double aindex, aindm1, P_floor;
     uint64_t step_index;
     double noise_dt;
+    /* test hook: a unit stress impulse replaces the drawn noise field */
+    int imp_on, imp_comp;
+    uint32_t imp_stream;
+    uint64_t imp_id;
+    double imp_val;
     /* boundary traces: [face][a][b][nface][nQ] flattened per side */
     double *ext[6], *intr[6];
     /* face fluxes flux_x(nface,nx+1,ny,nz,nQ) etc. (src/flux.f90:11-13) */
@@ -295,6 +300,11 @@ int orc_apply_mask(orc_ctx *c, double *q) {
 
 int orc_set_step_index(orc_ctx *c, uint64_t step) { c->step_index = step; return 0; }
 int orc_set_noise_dt(orc_ctx *c, double dt) { c->noise_dt = dt; return 0; }
+int orc_set_noise_impulse(orc_ctx *c, int on, uint32_t stream, uint64_t id, int comp, double value) {
+    if (on && (stream < 1 || stream > 4 || comp < 0 || comp > 5)) return set_err(-1, "impulse stream or component out of range");
+    c->imp_on = on; c->imp_stream = stream; c->imp_id = id; c->imp_comp = comp; c->imp_val = value;
+    return 0;
+}
 
 int orc_tables(const orc_ctx *c, double *xq, double *w1, double *w2, double *w3, double *cbasis,
                double *bint, double *bxp, double *bxm, double *byp, double *bym, double *bzp,
@@ -1044,6 +1054,11 @@ static void exchange_and_bc(orc_ctx *c, const double *q) {
    id = point*N + linear cell/face id, counters 2*id and 2*id+1. */
 static void noise_stress(const orc_ctx *c, uint32_t stream, uint64_t id, double amp_eta, double amp_zeta,
                          double S[6]) {
+    if (c->imp_on) {
+        for (int e = 0; e < 6; e++) S[e] = 0.0;
+        if (stream == c->imp_stream && id == c->imp_id) S[c->imp_comp] = c->imp_val;
+        return;
+    }
     double g[8];
     orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 2 * id, g);
     orc_noise_normals(c->c.seed, c->c.realization, c->step_index, stream, 2 * id + 1, g + 4);

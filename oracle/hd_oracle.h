@@ -51,6 +51,10 @@ orc_ctx *orc_create(const orc_config *cfg);
 void orc_destroy(orc_ctx *c);
 int orc_set_step_index(orc_ctx *c, uint64_t step);    /* noise counter; orc_update advances it by one */
 int orc_set_noise_dt(orc_ctx *c, double dt);          /* dt of the noise amplitude in orc_rhs (default 1) */
+/* linear-response hook: while `on`, every point of the stochastic stress is
+   zero except component comp (xx,yy,zz,xy,xz,yz) of point id of a stream
+   (1..3 faces, 4 volume), which is `value`; amplitudes are not applied */
+int orc_set_noise_impulse(orc_ctx *c, int on, uint32_t stream, uint64_t id, int comp, double value);
 /* prescribed exterior state of an x-low inflow boundary: def(ny,nz,nface,nQ)
    in Fortran order (Qxlo_ext_def); rh, mx and en are read */
 int orc_set_inflow(orc_ctx *c, const double *def);

@@ -68,6 +68,7 @@ def _configure(L):
     L.orc_tables.argtypes = [C.c_void_p] + [dp] * 13
     L.orc_set_step_index.argtypes = [C.c_void_p, C.c_uint64]
     L.orc_set_noise_dt.argtypes = [C.c_void_p, C.c_double]
+    L.orc_set_noise_impulse.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_double]
     L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.orc_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, dp]
     L.orc_noise_stress.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double, dp]
@@ -175,6 +176,14 @@ class Oracle:
 
     def set_noise_dt(self, dt: float):
         self.L.orc_set_noise_dt(self.h, float(dt))
+
+    def set_noise_impulse(self, stream: int = 0, point_id: int = 0, comp: int = 0, value: float = 0.0,
+                          on: bool = True):
+        """Replace the drawn stochastic stress by one unit impulse (value in
+        component comp of point point_id of a stream; every other point zero)
+        until called with on=False.  For linear-response analysis."""
+        self._check(self.L.orc_set_noise_impulse(self.h, int(on), int(stream), int(point_id), int(comp),
+                                                 float(value)))
 
     def set_inflow(self, inflow_def):
         """x-low inflow exterior state, numpy (nQ, nface, nz, ny) C order

@@ -189,26 +189,40 @@ def test_ensemble_rejects_bad_input(torch_mod):
         Ensemble(base.replace(nQ=5, noise=False, ivis=0), [(4, 4, 4)]).set_launch(999)
 
 
-def test_structure_factor_fdt(torch_mod):
-    """Equilibrium static structure factors of the stochastic 10-moment
-    model at small k against the fluctuation-dissipation values
-    S_uT = S_uL = T/rho = 1 and S_rr = rho/gamma (stress noise only)."""
+def test_structure_factor_linear_response(torch_mod, oracle_mod):
+    """Static structure factors of 256 realizations over 2,000 steps against
+    the exact linear-response prediction of the discrete scheme
+    (tests/fdt_linear.py).  The prediction is built from the oracle's one-step
+    stencils and includes the approach to equilibrium from the quiescent
+    start.  The Roe flux is used because it is smooth at rest, so the
+    linearisation is exact for the velocity channels.  The HLLC flux is not
+    smooth at rest: the stress part of its momentum flux switches sides with
+    the sign of the contact speed.  S_rr is not gated: the density slopes and
+    the mean density (entropy modes) are neutral in the linearisation and are
+    damped only by the |u|-weighted contact term, which is nonlinear.  The
+    low-k bins must agree to 5%; sampling error is about 1%."""
     from hermeshd_amd.ensemble import Ensemble, StructureFactor
-    base, dims = fluct_ensemble(sizes=((48, 16), (16, 24)))
+    from tests import fdt_linear
+    kB, dt, equil, nsamp, every = 1e-4, 0.1, 1000, 100, 10
+    base, dims = fluct_ensemble(sizes=((256, 16),), kB=kB)
+    base = base.replace(riemann="roe")
     ens = Ensemble(base, dims)
-    ens.tune(ens.pack([ics.uniform(realization_cfg(base, dims, r), rho=1.0, T=1.0) for r in range(len(dims))]),
-             ens.zeros(), 0.1, reps=1)
     q = ens.pack([ics.uniform(realization_cfg(base, dims, r), rho=1.0, T=1.0) for r in range(len(dims))])
     q1 = ens.zeros()
-    sf = StructureFactor(ens, kB=1e-4)
-    print("[fdt] equilibrating 400 steps", flush=True)
-    ens.run(q, q1, None, 0.1, 400)
-    for n in range(80):
-        ens.run(q, q1, None, 0.1, 10)
+    sf = StructureFactor(ens, kB=kB)
+    print(f"[sk] {len(dims)} realizations of 16^3, {equil} + {nsamp}x{every} steps", flush=True)
+    ens.run(q, q1, None, dt, equil)
+    for n in range(nsamp):
+        ens.run(q, q1, None, dt, every)
         sf.sample(q)
     ens.check()
-    b = sf.binned([0.2, 0.6, 1.0, 1.4])
-    print("[fdt]", {k: np.round(v, 3).tolist() for k, v in b.items()}, flush=True)
-    assert np.all(np.abs(b["S_uT"][:2] - 1.0) < 0.15)
-    assert np.all(np.abs(b["S_uL"][:2] - 1.0) < 0.25)
-    assert np.all(b["S_rr"][:2] > 0.4) and np.all(b["S_rr"][:2] < 0.9)
+    edges = [0.2, 0.6, 1.0, 1.4]
+    b = sf.binned(edges)
+    model = fdt_linear.LinearModel(realization_cfg(base, dims, 0), dt)
+    pred = fdt_linear.binned_prediction(model, dims, edges, kB, equil, nsamp, every)
+    for key in ("S_uL", "S_uT", "S_rr"):
+        print(f"[sk] {key} measured {np.round(b[key], 4).tolist()} predicted {np.round(pred[key], 4).tolist()}",
+              flush=True)
+    assert np.array_equal(b["n"], pred["n"])
+    for key in ("S_uL", "S_uT"):
+        assert np.all(np.abs(b[key][:2] / pred[key][:2] - 1.0) < 0.05), key
