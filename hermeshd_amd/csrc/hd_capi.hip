@@ -109,6 +109,7 @@ StageParams stage_params(const hd_ctx* c, const double* dt_dev, int combine, int
     sp.noise[3] = c->noise_v;
     sp.nfb = c->nfb;
     sp.inflow = c->inflow;
+    sp.rows_zfast = c->rows_zfast;
     return sp;
 }
 
@@ -310,6 +311,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         hd_destroy(c);
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
+    if (const char* ro = std::getenv("HD_STAGE_ROWS")) c->rows_zfast = std::strcmp(ro, "yz") != 0;
     // workgroup size of the fused stage kernel: 64, 128 or 256 threads
     if (const char* bs = std::getenv("HD_STAGE_BLOCK")) {
         const int v = std::atoi(bs);

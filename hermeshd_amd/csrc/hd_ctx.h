@@ -57,6 +57,9 @@ struct hd_ctx {
     // (x not periodic, nx not a multiple of 64 up to 256, or too much LDS);
     // HD_STAGE_XS=0 switches it off
     unsigned xs_bytes = 0;
+    // workgroup order of the row-shared kernel: z fastest (HD_STAGE_ROWS=zy, the
+    // default) or y fastest (HD_STAGE_ROWS=yz)
+    int rows_zfast = 1;
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage

@@ -110,6 +110,7 @@ StageParams ens_params(const hd_ens* e, const double* dt_dev, int combine, int m
     sp.err = e->err;
     sp.bad = e->bad;
     sp.limit_out = 0;
+    sp.rows_zfast = 0;  // the ensemble kernels walk (realization, tile) work lists
     sp.lp = e->lp;
     for (int d = 0; d < 4; d++) sp.noise[d] = nullptr;  // set per realization by the kernel
     sp.nfb = 0;

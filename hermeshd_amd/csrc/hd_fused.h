@@ -758,18 +758,25 @@ __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, c
 // One periodic row of nx cells per workgroup (nx a multiple of 64, at most
 // 256): the x faces are evaluated once and shared through LDS
 // (NF*NQ*nx doubles of dynamic shared memory).  The grid covers the rows
-// exactly, so every thread reaches the barrier.
+// exactly, so every thread reaches the barrier.  Each XCD takes a contiguous
+// range of workgroups; with rows_zfast the rows run with z fastest, so that
+// a row's z neighbours are the adjacent workgroups and its y neighbours nz
+// workgroups away, both inside the window of rows in flight on that XCD
+// (whose L2 then serves the neighbour modes); otherwise y runs fastest.
 template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_stage_xs(const double* __restrict__ qin, const double* __restrict__ qn,
                                                   double* __restrict__ qout, Grid g, Phys ph, StageParams sp) {
     extern __shared__ double xs[];
-    const long row = (long)xcd_block();
+    const long r = (long)xcd_block();
+    const int j = sp.rows_zfast ? (int)(r / g.nz) : (int)(r % g.ny);
+    const int k = sp.rows_zfast ? (int)(r % g.nz) : (int)(r / g.ny);
+    const long row = (long)j + (long)g.ny * k;
     const long c = row * g.nx + threadIdx.x;
     CellPos cp;
     cp.c = c;
     cp.i = (int)threadIdx.x;
-    cp.j = (int)(row % g.ny);
-    cp.k = (int)(row / g.ny);
+    cp.j = j;
+    cp.k = k;
     stage_cell<NB, QP, NQ, RM, NZ, true>(qin, qn, qout, g, ph, sp, cp, c, xs);
 }
 

@@ -278,6 +278,7 @@ struct StageParams {
     const float* noise[4];   // stochastic stress of the step (single precision): x, y, z face families, volume points
     long nfb;                // face lattice size (nx+1)(ny+1)(nz+1)
     const double* inflow;    // x-low inflow exterior state (ny,nz,nface,nQ), Fortran order
+    int rows_zfast;          // row-shared stage kernel: workgroup order with z fastest (else y fastest)
 };
 
 // ---------------------------------------------------------------------------
