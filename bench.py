@@ -239,6 +239,60 @@ def taylor_green_leg(n=192, steps=4, adj_n=48, adj_steps=20, checkpoint_every=5)
                         "ratio": round(tb / tf, 3), "KE": J, "norm_dKE_du0": gn}}
 
 
+def taylor_green_run(n, t_end, check_every=50, report_s=30.0):
+    """The 13-moment Taylor-Green vortex (taylor_green_3d(n)) stepped with the
+    reference's step semantics until t >= t_end.  One JSON line every
+    report_s seconds and at the end: time, steps, the mean kinetic energy of
+    the cell averages, its decay rate since the last report, and the drift of
+    the total mass, momentum and energy (relative to the totals of |.|).  The
+    decay rate at early times is compared with nu <|grad u|^2> = 3/(4 Re) of
+    the initial field (the incompressible Navier-Stokes dissipation rate)."""
+    import numpy as np
+    import torch
+    from hermeshd_amd import ics
+    from hermeshd_amd.config import taylor_green_3d
+    from hermeshd_amd.solver import Solver
+    dev = torch.device("cuda:0")
+    cfg = taylor_green_3d(n)
+    s = Solver(cfg, device=dev)
+    q = torch.from_numpy(ics.taylor_green(cfg)).to(dev)
+    q1 = torch.zeros_like(q)
+    dV = (cfg.lx / cfg.nx) * (cfg.ly / cfg.ny) * (cfg.lz / cfg.nz)
+
+    def diag():
+        m = q[0]
+        tot = (m[:5].reshape(5, -1).sum(1) * dV).cpu().numpy()
+        ke = float((0.5 * (m[1] ** 2 + m[2] ** 2 + m[3] ** 2) / m[0]).mean())
+        return tot, ke
+    tot0, ke0 = diag()
+    scale = (q[0, :5].abs().reshape(5, -1).sum(1) * dV).cpu().numpy()
+    eps0 = 0.75 / 1600.0
+    log(f"tgv-run: {n}^3, t_end {t_end}, KE0 {ke0:.6f}, nu<|grad u|^2> {eps0:.4e}")
+    steps, t, t_last, ke_last = 0, 0.0, 0.0, ke0
+    w0 = w_last = time.perf_counter()
+    while t < t_end:
+        for _ in range(check_every):
+            s.step(q, q1)
+        steps += check_every
+        t = float(s.t)
+        now = time.perf_counter()
+        if now - w_last >= report_s or t >= t_end:
+            s.check()
+            tot, ke = diag()
+            rec = {"n": n, "t": round(t, 6), "steps": steps, "KE": ke,
+                   "dKE_dt": (ke - ke_last) / (t - t_last), "rel_drift": ((tot - tot0) / scale).tolist(),
+                   "wall_s": round(now - w0, 1)}
+            print(json.dumps(rec), flush=True)
+            w_last, t_last, ke_last = now, t, ke
+    s.check()
+    tot, ke = diag()
+    wall = time.perf_counter() - w0
+    return {"workload": f"13-moment Taylor-Green vortex {n}^3, Re 1600, Pr 0.71, nbasis 10, Roe, limiter, SSP-RK3",
+            "t": t, "steps": steps, "KE0": ke0, "KE": ke, "mean_dKE_dt": (ke - ke0) / t, "nu_grad_u2_t0": eps0,
+            "max_rel_drift": float(np.max(np.abs((tot - tot0) / scale))), "ms_per_step": wall / steps * 1e3,
+            "wall_s": round(wall, 1)}
+
+
 def ensemble_leg(steps=20):
     """Batched ensemble (96x16^3 + 96x24^3 + 64x32^3 periodic boxes, nbasis 4,
     nQ 11, stochastic stress, HLLC, SSP-RK2, dt 0.1, graph-replayed steps,
@@ -305,12 +359,18 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-tgv", action="store_true", help="skip the 13-moment Taylor-Green leg")
+    ap.add_argument("--tgv-run", type=int, default=0, metavar="N",
+                    help="only run the 13-moment Taylor-Green vortex on N^3 cells to --tgv-t and print its diagnostics")
+    ap.add_argument("--tgv-t", type=float, default=1.0)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the grid would shard "
               "by slabs with one face-trace exchange per stage); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.tgv_run:
+        print(json.dumps({"tgv_run": taylor_green_run(args.tgv_run, args.tgv_t)}), flush=True)
+        return
 
     from hermeshd_amd import ics
     from hermeshd_amd.config import poiseuille_2d, sod_1d, vortex_3d
