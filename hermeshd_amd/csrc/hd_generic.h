@@ -529,6 +529,234 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 }
 
 // ---------------------------------------------------------------------------
+// Cell kernel of the 13-moment model, one field per wave: a workgroup is 13
+// waves over the same 64 consecutive cells, and wave f computes field f (its
+// volume integral, face lift, source and update).  A cell's 130 modes and 156
+// volume accumulators never sit in one lane: each lane holds its field's 10
+// modes and 12 accumulators, and the 13 point values of each volume point are
+// exchanged through LDS.  Loads and
+// stores of a wave touch one field of 64 consecutive cells.  The arithmetic
+// of every value is that of k_cell.  The volume points are exchanged one
+// plane of the rule (QP*QP points) at a time.
+// ---------------------------------------------------------------------------
+// x, y and z flux of field f of the 13-moment model at one point
+// (flux_point13, one field at a time; f is uniform over the wave)
+__device__ __forceinline__ void flux13_field(const Phys& ph, const double* Q, int f, double& Fx, double& Fy,
+                                             double& Fz) {
+    const double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
+    const double T_xx = Q[T_PXX], T_yy = Q[T_PYY], T_zz = -(T_xx + T_yy);
+    const double T_xy = Q[T_PXY], T_xz = Q[T_PXZ], T_yz = Q[T_PYZ];
+    const double dni = 1. / dn;
+    const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
+    const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
+    double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
+    if (P < ph.P_floor) P = ph.P_floor;
+    switch (f) {
+        case RH: Fx = M_x; Fy = M_y; Fz = M_z; break;
+        case MX: Fx = M_x * vx + P + T_xx; Fy = M_x * vy + T_xy; Fz = M_x * vz + T_xz; break;
+        case MY: Fx = M_y * vx + T_xy; Fy = M_y * vy + P + T_yy; Fz = M_y * vz + T_yz; break;
+        case MZ: Fx = M_z * vx + T_xz; Fy = M_z * vy + T_yz; Fz = M_z * vz + P + T_zz; break;
+        case EN:
+            Fx = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz + Q[T_QX];
+            Fy = (Ener + P) * vy + T_yy * vy + T_xy * vx + T_yz * vz + Q[T_QY];
+            Fz = (Ener + P) * vz + T_zz * vz + T_xz * vx + T_yz * vy + Q[T_QZ];
+            break;
+        case T_PXX: Fx = ph.c4d3cv * vx; Fy = -ph.c2d3cv * vy; Fz = -ph.c2d3cv * vz; break;
+        case T_PYY: Fx = -ph.c2d3cv * vx; Fy = ph.c4d3cv * vy; Fz = -ph.c2d3cv * vz; break;
+        case T_PXY: Fx = ph.colvis * vy; Fy = ph.colvis * vx; Fz = 0.; break;
+        case T_PXZ: Fx = ph.colvis * vz; Fy = 0.; Fz = ph.colvis * vx; break;
+        case T_PYZ: Fx = 0.; Fy = ph.colvis * vz; Fz = ph.colvis * vy; break;
+        default: {
+            const double ct = ph.colq * (P * dni);
+            Fx = f == T_QX ? ct : 0.;
+            Fy = f == T_QY ? ct : 0.;
+            Fz = f == T_QZ ? ct : 0.;
+        }
+    }
+}
+
+constexpr int CELL13_WAVES = 13;
+
+template <int NB, int QP>
+__global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
+    const double* __restrict__ qin, const double* __restrict__ qn, double* __restrict__ qout,
+    const double* __restrict__ flx, const double* __restrict__ fly, const double* __restrict__ flz,
+    const Tables* __restrict__ T, Grid g, Phys ph, StageParams sp) {
+    constexpr int NQ = 13, NF = QP * QP, NPG = QP * QP * QP;
+    constexpr int NS = (NB > 4) ? 12 : 3;
+    constexpr int NCH = NF;  // volume points per exchange: one plane of the 3D rule
+    __shared__ double qp_s[NQ][NCH][64];
+    const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const long c = (long)blockIdx.x * 64 + lane;
+    const bool live = c < g.N;
+    const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
+    const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));
+    double mf[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) mf[r] = qin[qoff(g, cc, f, r)];
+    // ---- volume integral: every wave reaches every barrier ----
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) acc[s] = 0.;
+    // volume integral, NCH points per exchange: each wave writes its field's
+    // point values, then reads the 13 values of each point; every wave
+    // reaches every barrier
+#pragma unroll 1
+    for (int p0 = 0; p0 < NPG; p0 += NCH) {
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            const double* b = T->bint[p0 + q];
+            double v = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) v += b[r] * mf[r];
+            qp_s[f][q][lane] = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < NCH; q++) {
+            const int p = p0 + q;
+            const double* b = T->bint[p];
+            double Qp[NQ];
+#pragma unroll
+            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][q][lane];
+            double Fx, Fy, Fz;
+            flux13_field(ph, Qp, f, Fx, Fy, Fz);
+            const double w = T->w3[p];
+            acc[0] += w * Fx;
+            acc[1] += w * Fy;
+            acc[2] += w * Fz;
+            if constexpr (NB > 4) {
+                acc[3] += w * b[KZ] * Fy;
+                acc[4] += w * b[KY] * Fz;
+                acc[5] += w * b[KZ] * Fx;
+                acc[6] += w * b[KX] * Fz;
+                acc[7] += w * b[KY] * Fx;
+                acc[8] += w * b[KX] * Fy;
+                if constexpr (NB == 10) {
+                    acc[9] += w * 3. * b[KX] * Fx;
+                    acc[10] += w * 3. * b[KY] * Fy;
+                    acc[11] += w * 3. * b[KZ] * Fz;
+                } else {
+                    acc[9] += w * b[KYZ] * Fx;
+                    acc[10] += w * b[KZX] * Fy;
+                    acc[11] += w * b[KXY] * Fz;
+                }
+            }
+        }
+        __syncthreads();  // the next plane overwrites qp_s
+    }
+    if (!live) return;  // no barrier below
+    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+    double intr[NB];
+    intr[0] = 0.;
+    intr[KX] = 0.25 * T->cb[KX] * dxi * acc[0];
+    intr[KY] = 0.25 * T->cb[KY] * dyi * acc[1];
+    intr[KZ] = 0.25 * T->cb[KZ] * dzi * acc[2];
+    if constexpr (NB > 4) {
+        intr[KYZ] = 0.25 * T->cb[KYZ] * (dyi * acc[3] + dzi * acc[4]);
+        intr[KZX] = 0.25 * T->cb[KZX] * (dxi * acc[5] + dzi * acc[6]);
+        intr[KXY] = 0.25 * T->cb[KXY] * (dxi * acc[7] + dyi * acc[8]);
+        if constexpr (NB == 10) {
+            intr[KXX] = 0.25 * T->cb[KXX] * (dxi * acc[9]);
+            intr[KYY] = 0.25 * T->cb[KYY] * (dyi * acc[10]);
+            intr[KZZ] = 0.25 * T->cb[KZZ] * (dzi * acc[11]);
+        } else {
+            intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[9] + dyi * acc[10] + dzi * acc[11]);
+        }
+    }
+    // face lift of field f, one direction at a time (2*NF face values live):
+    // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
+    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
+    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
+    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+    const long nfl[3] = {(long)(g.nx + 1) * g.ny * g.nz, (long)g.nx * (g.ny + 1) * g.nz,
+                         (long)g.nx * g.ny * (g.nz + 1)};
+    const long fmi[3] = {fxm, fym, fzm};
+    const long fpi[3] = {fxm + 1, fym + g.nx, fzm + (long)g.nx * g.ny};
+    const double di[3] = {dxi, dyi, dzi};
+    const double* const fls[3] = {flx, fly, flz};
+    double t[NB];
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+        const double* fl = fls[d];
+        double fm[NF], fp[NF];
+#pragma unroll
+        for (int p = 0; p < NF; p++) {
+            fm[p] = fl[((long)f * NF + p) * nfl[d] + fmi[d]];
+            fp[p] = fl[((long)f * NF + p) * nfl[d] + fpi[d]];
+        }
+        const int FM = 2 * d, FP = 2 * d + 1;  // F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP
+        double s0 = 0.;
+#pragma unroll
+        for (int p = 0; p < NF; p++) s0 += T->w2[p] * (fp[p] - fm[p]);
+        t[0] = d == 0 ? di[d] * s0 : t[0] + di[d] * s0;
+#pragma unroll
+        for (int r = 1; r < NB; r++) {
+            double sr = 0.;
+#pragma unroll
+            for (int p = 0; p < NF; p++) sr += T->w2[p] * (T->bfc[FP][p][r] * fp[p] - T->bfc[FM][p][r] * fm[p]);
+            t[r] = d == 0 ? di[d] * sr : t[r] + di[d] * sr;
+        }
+    }
+    double glf[NB];
+    glf[0] = 0.25 * t[0];
+#pragma unroll
+    for (int r = 1; r < NB; r++) glf[r] = 0.25 * T->cb[r] * t[r] - intr[r];
+    // ---- source: explicit relaxation (ivis 0), body force ----
+    double src[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) src[r] = 0.;
+    const double rate = f >= T_QX ? sp.coll_q : sp.coll;
+    if (sp.ivis == 0 && f >= T_PXX && rate != 0.) {
+#pragma unroll 1
+        for (int r = 0; r < NB; r++) {
+            double s = 0.;
+#pragma unroll 1
+            for (int p = 0; p < NPG; p++) {
+                double qv = 0.;
+#pragma unroll
+                for (int rr = 0; rr < NB; rr++) qv += T->bint[p][rr] * mf[rr];
+                s += T->bwint[p][r] * (-rate * qv);
+            }
+            src[r] = 0.125 * T->cb[r] * s;
+        }
+    }
+    if (f >= MX && f <= MZ && sp.bf[f - MX] != 0.)
+#pragma unroll
+        for (int r = 0; r < NB; r++) src[r] += sp.bf[f - MX] * qin[qoff(g, c, RH, r)];
+    if (f == EN && (sp.bf[0] != 0. || sp.bf[1] != 0. || sp.bf[2] != 0.))
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+            src[r] += sp.bf[0] * qin[qoff(g, c, MX, r)] + sp.bf[1] * qin[qoff(g, c, MY, r)] +
+                      sp.bf[2] * qin[qoff(g, c, MZ, r)];
+    // ---- update, SSP combination, NaN check of mode 1 ----
+    if (sp.mode == 1) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) qout[qoff(g, c, f, r)] = -(glf[r] - src[r]);
+        return;
+    }
+    const double dt = *sp.dt;
+    const double fac = f >= T_QX ? 1. / (1. + dt * sp.coll_q) : 1. / (1. + dt * sp.coll);
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        double v = mf[r] - dt * (glf[r] - src[r]);
+        if (sp.ivis == 1 && f >= T_PXX) v = fac * v;
+        const long o = qoff(g, c, f, r);
+        if (r == 0 && v != v) bad = true;
+        if (sp.combine == 1) v = 0.5 * (qn[o] + v);
+        else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
+        else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
+        qout[o] = v;
+    }
+    if (bad) {
+        atomicOr(sp.err, 1);
+        atomicMin(sp.bad, (unsigned long long)c);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // get_min_dt (src/hermeshd.f90:145-198): per-block max of the wave speed over
 // cells with rho > rh_mult*rh_floor, then a single-block finish that adds the
 // sound speed of the last cell, as the reference does.

@@ -21,6 +21,13 @@ struct Impl {
         hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph, c->inflow);
     }
     static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
+        if constexpr (NQ == 13) {
+            // one field per wave, 64 cells per workgroup (every wave of a
+            // workgroup reaches its barriers: the grid is padded, not clipped)
+            hipLaunchKernelGGL((k_cell13<NB, QP>), dim3(blocks_for(c->N, 64)), dim3(CELL13_WAVES * 64), 0, s, qin,
+                               qn, qout, c->fl[0], c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
+            return;
+        }
         hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
                            c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
     }
