@@ -218,7 +218,8 @@ def taylor_green_3d(n: int = 192, Re: float = 1600.0, Pr: float = 0.71, gamma: f
                     riemann: str = "roe") -> HDConfig:
     """13-moment Taylor-Green vortex: n^3 periodic cells on [0, 2 pi]^3 (the
     grid is centred on the origin), complete-quadratic basis, rho = 1,
-    p = 100/gamma (Mach 0.1 for |u| = 1), Roe flux, limiter on, SSP-RK3.
+    mean pressure 100/gamma (Mach 0.1 for |u| = 1), Roe flux, limiter on,
+    SSP-RK3.
     Viscosity mu = 1/Re (unit velocity and length): the stress relaxes at
     coll = p/mu with colvis = coll*mu = p; the heat flux relaxes at the same
     rate, coll_q = coll, with conductivity kappa = mu c_p/Pr, c_p =

@@ -136,8 +136,11 @@ def uniform(cfg: HDConfig, rho: float = 1.0, T: float = 1.0, v=(0.0, 0.0, 0.0)) 
 
 def taylor_green(cfg: HDConfig, rho: float = 1.0, noise: float = 1e-4, seed: int = 1234) -> np.ndarray:
     """Taylor-Green vortex on [0, L]^3 (coordinates shifted from the centred
-    grid): u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
-    p = 100/gamma, plus noise*N(0,1) on each velocity component."""
+    grid): u = sin x cos y cos z, v = -cos x sin y cos z, w = 0, uniform
+    density, and the pressure of the incompressible vortex
+    p = 100/gamma + (rho/16)(cos 2x + cos 2y)(cos 2z + 2), which balances the
+    initial momentum flux so that no acoustic transient starts (Mach 0.1 for
+    |u| = 1); plus noise*N(0,1) on each velocity component."""
     q = background(cfg)
     xc, yc, zc = cell_centres(cfg)
     x = (xc + cfg.lx / 2)[None, None, :]
@@ -148,7 +151,8 @@ def taylor_green(cfg: HDConfig, rho: float = 1.0, noise: float = 1e-4, seed: int
     u = np.sin(x) * np.cos(y) * np.cos(z) + noise * rng.standard_normal(shp)
     v = -np.cos(x) * np.sin(y) * np.cos(z) + noise * rng.standard_normal(shp)
     w = noise * rng.standard_normal(shp)
-    _set_primitive(q, cfg, np.full(shp, rho), u, v, w, np.full(shp, 100.0 / cfg.gamma))
+    p = 100.0 / cfg.gamma + (rho / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    _set_primitive(q, cfg, np.full(shp, rho), u, v, w, np.broadcast_to(p, shp))
     return q
 
 
