@@ -537,20 +537,31 @@ __global__ __launch_bounds__(128) void k_cell(const double* __restrict__ qin, co
 // exchanged through LDS.  Loads and
 // stores of a wave touch one field of 64 consecutive cells.  The arithmetic
 // of every value is that of k_cell.  The volume points are exchanged one
-// plane of the rule (QP*QP points) at a time.
+// plane of the rule (QP*QP points) at a time, and their primitives (1/rho,
+// velocity, pressure) are computed once per point and shared the same way.
 // ---------------------------------------------------------------------------
-// x, y and z flux of field f of the 13-moment model at one point
-// (flux_point13, one field at a time; f is uniform over the wave)
-__device__ __forceinline__ void flux13_field(const Phys& ph, const double* Q, int f, double& Fx, double& Fy,
-                                             double& Fz) {
+// Primitives of the 13-moment flux at one point (flux_point13): 1/rho,
+// velocity, the floored pressure and the heat-flux flux col_q*T.
+constexpr int NPRIM13 = 6;
+__device__ __forceinline__ void prim13(const Phys& ph, const double* Q, double* pr) {
     const double dn = Q[RH], M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
-    const double T_xx = Q[T_PXX], T_yy = Q[T_PYY], T_zz = -(T_xx + T_yy);
-    const double T_xy = Q[T_PXY], T_xz = Q[T_PXZ], T_yz = Q[T_PYZ];
     const double dni = 1. / dn;
     const double vx = M_x * dni, vy = M_y * dni, vz = M_z * dni;
     const double smsq = M_x * M_x + M_y * M_y + M_z * M_z;
     double P = ph.gm1 * (Ener - 0.5 * dni * smsq);
     if (P < ph.P_floor) P = ph.P_floor;
+    pr[0] = dni; pr[1] = vx; pr[2] = vy; pr[3] = vz; pr[4] = P;
+    pr[5] = ph.colq * (P * dni);
+}
+
+// x, y and z flux of field f at one point from its state and primitives
+// (flux_point13, one field at a time; f is uniform over the wave)
+__device__ __forceinline__ void flux13_field(const Phys& ph, const double* Q, const double* pr, int f, double& Fx,
+                                             double& Fy, double& Fz) {
+    const double M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
+    const double T_xx = Q[T_PXX], T_yy = Q[T_PYY], T_zz = -(T_xx + T_yy);
+    const double T_xy = Q[T_PXY], T_xz = Q[T_PXZ], T_yz = Q[T_PYZ];
+    const double vx = pr[1], vy = pr[2], vz = pr[3], P = pr[4], ct = pr[5];
     switch (f) {
         case RH: Fx = M_x; Fy = M_y; Fz = M_z; break;
         case MX: Fx = M_x * vx + P + T_xx; Fy = M_x * vy + T_xy; Fz = M_x * vz + T_xz; break;
@@ -566,13 +577,42 @@ __device__ __forceinline__ void flux13_field(const Phys& ph, const double* Q, in
         case T_PXY: Fx = ph.colvis * vy; Fy = ph.colvis * vx; Fz = 0.; break;
         case T_PXZ: Fx = ph.colvis * vz; Fy = 0.; Fz = ph.colvis * vx; break;
         case T_PYZ: Fx = 0.; Fy = ph.colvis * vz; Fz = ph.colvis * vy; break;
-        default: {
-            const double ct = ph.colq * (P * dni);
+        default:
             Fx = f == T_QX ? ct : 0.;
             Fy = f == T_QY ? ct : 0.;
             Fz = f == T_QZ ? ct : 0.;
-        }
     }
+}
+
+// Face lift of one field in direction D (k_cell13): t[r] accumulates
+// di*sum_p w2_p (bfc[+][p][r] fl(p,+) - bfc[-][p][r] fl(p,-)) over the
+// directions, with the sums of k_cell.  The point loop stays rolled, so each
+// point's table values are loaded as they are used; the face values of the
+// next point are loaded one iteration ahead (the last iteration reloads its
+// own point, in bounds).
+template <int NB, int NF, int D>
+__device__ __forceinline__ void lift13_dir(const double* __restrict__ fl, long nfl, long fm, long fp, double di,
+                                           int f, const Tables* __restrict__ T, double (&t)[NB]) {
+    constexpr int FM = 2 * D, FP = 2 * D + 1;  // F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP
+    double sr[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) sr[r] = 0.;
+    double vm = fl[((long)f * NF) * nfl + fm];
+    double vp = fl[((long)f * NF) * nfl + fp];
+#pragma unroll 1
+    for (int p = 0; p < NF; p++) {
+        const int pn = p + 1 < NF ? p + 1 : p;
+        const double vmn = fl[((long)f * NF + pn) * nfl + fm];
+        const double vpn = fl[((long)f * NF + pn) * nfl + fp];
+        const double w = T->w2[p];
+        sr[0] += w * (vp - vm);
+#pragma unroll
+        for (int r = 1; r < NB; r++) sr[r] += w * (T->bfc[FP][p][r] * vp - T->bfc[FM][p][r] * vm);
+        vm = vmn;
+        vp = vpn;
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) t[r] = D == 0 ? di * sr[r] : t[r] + di * sr[r];
 }
 
 constexpr int CELL13_WAVES = 13;
@@ -585,7 +625,9 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
     constexpr int NQ = 13, NF = QP * QP, NPG = QP * QP * QP;
     constexpr int NS = (NB > 4) ? 12 : 3;
     constexpr int NCH = NF;  // volume points per exchange: one plane of the 3D rule
+    static_assert(NCH <= CELL13_WAVES, "one wave per point computes the primitives");
     __shared__ double qp_s[NQ][NCH][64];
+    __shared__ double pr_s[NCH][NPRIM13][64];
     const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
     const long c = (long)blockIdx.x * 64 + lane;
@@ -613,15 +655,27 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
             qp_s[f][q][lane] = v;
         }
         __syncthreads();
+        // the primitives of point f of the plane, once per point
+        if (f < NCH) {
+            double Qp[NQ], pr[NPRIM13];
+#pragma unroll
+            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][f][lane];
+            prim13(ph, Qp, pr);
+#pragma unroll
+            for (int e = 0; e < NPRIM13; e++) pr_s[f][e][lane] = pr[e];
+        }
+        __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < NCH; q++) {
             const int p = p0 + q;
             const double* b = T->bint[p];
-            double Qp[NQ];
+            double Qp[NQ], pr[NPRIM13];
 #pragma unroll
             for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][q][lane];
+#pragma unroll
+            for (int e = 0; e < NPRIM13; e++) pr[e] = pr_s[q][e][lane];
             double Fx, Fy, Fz;
-            flux13_field(ph, Qp, f, Fx, Fy, Fz);
+            flux13_field(ph, Qp, pr, f, Fx, Fy, Fz);
             const double w = T->w3[p];
             acc[0] += w * Fx;
             acc[1] += w * Fy;
@@ -644,7 +698,7 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
                 }
             }
         }
-        __syncthreads();  // the next plane overwrites qp_s
+        __syncthreads();  // the next plane overwrites qp_s and pr_s
     }
     if (!live) return;  // no barrier below
     const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
@@ -665,40 +719,15 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
             intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[9] + dyi * acc[10] + dzi * acc[11]);
         }
     }
-    // face lift of field f, one direction at a time (2*NF face values live):
+    // face lift of field f, one direction at a time:
     // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
     const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
     const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
     const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
-    const long nfl[3] = {(long)(g.nx + 1) * g.ny * g.nz, (long)g.nx * (g.ny + 1) * g.nz,
-                         (long)g.nx * g.ny * (g.nz + 1)};
-    const long fmi[3] = {fxm, fym, fzm};
-    const long fpi[3] = {fxm + 1, fym + g.nx, fzm + (long)g.nx * g.ny};
-    const double di[3] = {dxi, dyi, dzi};
-    const double* const fls[3] = {flx, fly, flz};
     double t[NB];
-#pragma unroll 1
-    for (int d = 0; d < 3; d++) {
-        const double* fl = fls[d];
-        double fm[NF], fp[NF];
-#pragma unroll
-        for (int p = 0; p < NF; p++) {
-            fm[p] = fl[((long)f * NF + p) * nfl[d] + fmi[d]];
-            fp[p] = fl[((long)f * NF + p) * nfl[d] + fpi[d]];
-        }
-        const int FM = 2 * d, FP = 2 * d + 1;  // F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP
-        double s0 = 0.;
-#pragma unroll
-        for (int p = 0; p < NF; p++) s0 += T->w2[p] * (fp[p] - fm[p]);
-        t[0] = d == 0 ? di[d] * s0 : t[0] + di[d] * s0;
-#pragma unroll
-        for (int r = 1; r < NB; r++) {
-            double sr = 0.;
-#pragma unroll
-            for (int p = 0; p < NF; p++) sr += T->w2[p] * (T->bfc[FP][p][r] * fp[p] - T->bfc[FM][p][r] * fm[p]);
-            t[r] = d == 0 ? di[d] * sr : t[r] + di[d] * sr;
-        }
-    }
+    lift13_dir<NB, NF, 0>(flx, (long)(g.nx + 1) * g.ny * g.nz, fxm, fxm + 1, dxi, f, T, t);
+    lift13_dir<NB, NF, 1>(fly, (long)g.nx * (g.ny + 1) * g.nz, fym, fym + g.nx, dyi, f, T, t);
+    lift13_dir<NB, NF, 2>(flz, (long)g.nx * g.ny * (g.nz + 1), fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
     double glf[NB];
     glf[0] = 0.25 * t[0];
 #pragma unroll
