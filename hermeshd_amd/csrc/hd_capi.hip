@@ -312,6 +312,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         return fail(HD_ERR_HIP, "initialising context buffers failed");
     }
     if (const char* ro = std::getenv("HD_STAGE_ROWS")) c->rows_zfast = std::strcmp(ro, "yz") != 0;
+    if (const char* lm = std::getenv("HD_LIMITER")) c->table_limiter = std::strcmp(lm, "table") == 0;
+    if (const char* fc = std::getenv("HD_FACES")) c->table_faces = std::strcmp(fc, "table") == 0;
     // workgroup size of the fused stage kernel: 64, 128 or 256 threads
     if (const char* bs = std::getenv("HD_STAGE_BLOCK")) {
         const int v = std::atoi(bs);

@@ -60,6 +60,10 @@ struct hd_ctx {
     // workgroup order of the row-shared kernel: z fastest (HD_STAGE_ROWS=zy, the
     // default) or y fastest (HD_STAGE_ROWS=yz)
     int rows_zfast = 1;
+    // table-driven limiter kernel instead of fz::k_limit (HD_LIMITER=table)
+    int table_limiter = 0;
+    // one-thread-per-face k_face_flux for periodic 13-moment grids too (HD_FACES=table)
+    int table_faces = 0;
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage

@@ -734,13 +734,6 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ qin, const
     }
 }
 
-// Workgroup b runs on XCD b mod 8.  Giving each XCD a contiguous eighth of the
-// cells keeps a cell's y and z neighbours (nx and nx*ny cells away) in the
-// same XCD's L2 instead of another's.
-__device__ __forceinline__ unsigned xcd_block() {
-    const unsigned b = blockIdx.x, G = gridDim.x;
-    return (G & 7u) == 0 ? (b & 7u) * (G >> 3) + (b >> 3) : b;
-}
 
 template <int NB, int QP, int NQ, int RM, bool NZ>
 __global__ __launch_bounds__(256) void k_stage(const double* __restrict__ qin, const double* __restrict__ qn,
@@ -778,6 +771,70 @@ __global__ __launch_bounds__(256) void k_stage_xs(const double* __restrict__ qin
     cp.j = j;
     cp.k = k;
     stage_cell<NB, QP, NQ, RM, NZ, true>(qin, qn, qout, g, ph, sp, cp, c, xs);
+}
+
+// Face fluxes of the 13-moment model on a grid periodic in every direction,
+// in k_face_flux's output layout fl[((f*NF + p)*nfaces) + face].  A workgroup
+// owns 64 consecutive faces of direction DIR, one wave per face point, so no
+// lane holds more than one point's states.  Phase 1 turns the modes of the
+// cells on both sides into face-polynomial coefficients (face_coeffs, NK per
+// field and side) in LDS, one (side, field, face) item per thread.  Phase 2
+// evaluates point p of every face on wave p (face_eval), then the physical
+// fluxes and the Riemann solver of k_face_flux.  Lanes past the last face
+// reuse the last face in phase 1 and store nothing.
+template <int NB, int QP, int RM, int DIR>
+__global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restrict__ q, double* __restrict__ fl,
+                                                        Grid g, Phys ph) {
+    constexpr int NQ = 13, NF = QP * QP, NK = face_nk<NB>();
+    __shared__ double ks[2][NQ][NK][64];
+    const int ni = g.nx + (DIR == 0), nj = g.ny + (DIR == 1);
+    const long nfaces = (long)ni * nj * (g.nz + (DIR == 2));
+    const long f0 = (long)xcd_block() * 64;  // XCD-contiguous ranges of faces
+    for (int it = (int)threadIdx.x; it < 2 * NQ * 64; it += NF * 64) {
+        const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
+        const long fid = f0 + l < nfaces ? f0 + l : nfaces - 1;
+        int ci = (int)(fid % ni), cj = (int)((fid / ni) % nj), ck = (int)(fid / ((long)ni * nj));
+        // side 0: the cell below the face (index a - 1, wrapped), side 1: above (index a, wrapped)
+        int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
+        const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+        if (side == 0) a = a == 0 ? n_ax - 1 : a - 1;
+        else if (a == n_ax) a = 0;
+        const long cell = (long)ci + (long)g.nx * ((long)cj + (long)g.ny * ck);
+        double m1[1][NB], K1[1][NK];
+#pragma unroll
+        for (int r = 0; r < NB; r++) m1[0][r] = q[((long)r * NQ + fld) * g.N + cell];
+        if (side == 0) face_coeffs<NB, 1, DIR, 1, NK>(m1, K1);
+        else face_coeffs<NB, 1, DIR, 0, NK>(m1, K1);
+#pragma unroll
+        for (int k = 0; k < NK; k++) ks[side][fld][k][l] = K1[0][k];
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int l = (int)(threadIdx.x & 63);
+    const long fid = f0 + l;
+    if (fid >= nfaces) return;
+    const auto& FP = face_poly<NB>();
+    double QL[NQ], QR[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        double sl = ks[0][f][0][l], sr = ks[1][f][0][l];
+#pragma unroll
+        for (int k = 1; k < NK; k++) {
+            sl += ks[0][f][k][l] * FP.b[p][k];
+            sr += ks[1][f][k][l] * FP.b[p][k];
+        }
+        QL[f] = sl;
+        QR[f] = sr;
+    }
+    double FL[NQ], FR[NQ], out[NQ];
+    gflux<NQ, DIR>(ph, QL, FL);
+    gflux<NQ, DIR>(ph, QR, FR);
+#pragma unroll
+    for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
+    if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL, QR, FL, FR, out);  // ideal gas: no last-state speed
+    if constexpr (RM == RIEMANN_ROE) roe_point<DIR, NQ>(ph, QL, QR, FL, FR, out);
+#pragma unroll
+    for (int f = 0; f < NQ; f++) fl[((long)f * NF + p) * nfaces + fid] = out[f];
 }
 
 }  // namespace fz

@@ -22,6 +22,14 @@ struct LimParams {
     double rh_floor, epsP, gm1;
 };
 
+// Workgroup b runs on XCD b mod 8.  Giving each XCD a contiguous eighth of the
+// cells keeps a cell's y and z neighbours (nx and nx*ny cells away) in the
+// same XCD's L2 instead of another's.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, G = gridDim.x;
+    return (G & 7u) == 0 ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+}
+
 __device__ __forceinline__ long qoff(const Grid& g, long cell, int f, int r) {
     return ((long)r * g.nQ + f) * g.N + cell;
 }
@@ -630,7 +638,7 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
     __shared__ double pr_s[NCH][NPRIM13][64];
     const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
-    const long c = (long)blockIdx.x * 64 + lane;
+    const long c = (long)xcd_block() * 64 + lane;  // XCD-contiguous ranges of cells
     const bool live = c < g.N;
     const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
     const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));

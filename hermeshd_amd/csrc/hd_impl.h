@@ -8,7 +8,9 @@ namespace hd {
 template <int NB, int QP, int NQ, int RM>
 struct Impl {
     static void limiter(hd_ctx* c, double* q, hipStream_t s) {
-        if (c->generic)
+        // compile-time face tables and an exact early-out for cells no floor
+        // can reach; the table-driven k_limiter stays for HD_LIMITER=table
+        if (c->generic && c->table_limiter)
             hipLaunchKernelGGL((k_limiter<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->dtab, c->grid, c->lp);
         else
             hipLaunchKernelGGL((fz::k_limit<NB, QP>), dim3(blocks_for(c->N, 256)), dim3(256), 0, s, q, c->grid, c->lp);
@@ -16,6 +18,18 @@ struct Impl {
     static void faces(hd_ctx* c, const double* q, hipStream_t s) {
         const Grid& g = c->grid;
         long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz, nfz = (long)g.nx * g.ny * (g.nz + 1);
+        if constexpr (NQ == 13) {
+            // all-periodic grids: one wave per face point (fz::k_face13)
+            bool periodic = !c->table_faces;
+            for (int b = 0; b < 6; b++) periodic = periodic && g.bc[b] == BC_PERIODIC;
+            if (periodic) {
+                constexpr int NT = QP * QP * 64;
+                hipLaunchKernelGGL((fz::k_face13<NB, QP, RM, 0>), dim3(blocks_for(nfx, 64)), dim3(NT), 0, s, q, c->fl[0], g, c->ph);
+                hipLaunchKernelGGL((fz::k_face13<NB, QP, RM, 1>), dim3(blocks_for(nfy, 64)), dim3(NT), 0, s, q, c->fl[1], g, c->ph);
+                hipLaunchKernelGGL((fz::k_face13<NB, QP, RM, 2>), dim3(blocks_for(nfz, 64)), dim3(NT), 0, s, q, c->fl[2], g, c->ph);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 0>), dim3(blocks_for(nfx, 128)), dim3(128), 0, s, q, c->fl[0], c->dtab, g, c->ph, c->inflow);
         hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 1>), dim3(blocks_for(nfy, 128)), dim3(128), 0, s, q, c->fl[1], c->dtab, g, c->ph, c->inflow);
         hipLaunchKernelGGL((k_face_flux<NB, QP, NQ, RM, 2>), dim3(blocks_for(nfz, 128)), dim3(128), 0, s, q, c->fl[2], c->dtab, g, c->ph, c->inflow);

@@ -638,3 +638,36 @@ def test_row_order_does_not_change_results(torch_mod):
         s.check()
         outs.append(q)
     assert torch_mod.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("name,env", [("tm13_nb4_roe_periodic", {"HD_FACES": "table"}),
+                                      ("tgv_nb10_roe_8", {"HD_FACES": "table", "HD_LIMITER": "table"}),
+                                      ("tenmom_nb4_hllc_periodic", {"HD_LIMITER": "table"})])
+def test_table_driven_variants(torch_mod, oracle_mod, name, env):
+    """The one-thread-per-face k_face_flux (HD_FACES=table; periodic 13-moment
+    grids otherwise take fz::k_face13) and the table-driven limiter
+    (HD_LIMITER=table; fz::k_limit otherwise) against the oracle."""
+    from hermeshd_amd.solver import Solver
+    cfg = CASES[name]
+    print(f"[table_variants] {name} {env}", flush=True)
+    q = ics.taylor_green(cfg) if name.startswith("tgv") else smooth_state(cfg, seed=5)
+    o = oracle_mod.Oracle(cfg.as_ints())
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        s = Solver(cfg)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    qd = _dev(q)
+    q1d = torch_mod.zeros_like(qd)
+    q1, q2 = np.zeros_like(q), np.zeros_like(q)
+    t, dt = np.array(0.0), np.array(0.0)
+    for n in range(3):
+        o.step(q, q1, q2, t, dt)
+        s.step(qd, q1d)
+    s.check()
+    assert field_rel_err(qd.cpu().numpy(), q) < TOL
