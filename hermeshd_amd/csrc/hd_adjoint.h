@@ -862,6 +862,309 @@ __global__ __launch_bounds__(256) void k_stage_vjp_split(const double* __restric
     if (ap.partial) block_partial(dcoll, ap.partial);
 }
 
+// ------------------------------------------------- 13-moment transpose ----
+// The transpose of a 13-moment stage on a grid periodic in every direction,
+// split as the forward stage is (fz::k_face13 + k_cell13), so that no lane
+// holds a whole cell: the face kernel writes, per face point, the cotangents
+// that the face flux sends to the traces of its two cells; the cell kernel,
+// one field per wave, adds the volume transpose and gathers those face
+// cotangents.  The sum over the terms is the fused k_stage_vjp's, evaluated
+// in another order.
+
+// One wave per face point over 64 consecutive faces of direction DIR (as
+// fz::k_face13): phase 1 stages both sides' trace coefficients (face_coeffs
+// of the limited modes q) and lifted-cotangent coefficients (lift_coeffs of
+// b*lam through the relaxation factors) in LDS; phase 2 forms the face
+// flux's cotangent mu at point p, reverses the Riemann solver and the
+// physical fluxes (face_vjp), and writes the left cell's trace cotangent to
+// bql and the right cell's to bqr, laid out as the forward face fluxes.
+template <int NB, int QP, int RM, int DIR>
+__global__ __launch_bounds__(QP * QP * 64) void k_face13_vjp(const double* __restrict__ q,
+                                                            const double* __restrict__ lam, double* __restrict__ bql,
+                                                            double* __restrict__ bqr, Grid g, Phys ph, AdjParams ap) {
+    constexpr int NQ = 13, NF = QP * QP, NK = fz::face_nk<NB>();
+    __shared__ double ks[2][NQ][NK][64];
+    __shared__ double ls[2][NQ][NK][64];
+    const double dt = *ap.dt;
+    const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+    const double faciq = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll_q) : 1.0;
+    const int ni = g.nx + (DIR == 0), nj = g.ny + (DIR == 1);
+    const long nfaces = (long)ni * nj * (g.nz + (DIR == 2));
+    const long f0 = (long)xcd_block() * 64;
+    for (int it = (int)threadIdx.x; it < 2 * NQ * 64; it += NF * 64) {
+        const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
+        const long fid = f0 + l < nfaces ? f0 + l : nfaces - 1;
+        int ci = (int)(fid % ni), cj = (int)((fid / ni) % nj), ck = (int)(fid / ((long)ni * nj));
+        int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
+        const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+        if (side == 0) a = a == 0 ? n_ax - 1 : a - 1;
+        else if (a == n_ax) a = 0;
+        const long cell = (long)ci + (long)g.nx * ((long)cj + (long)g.ny * ck);
+        const double fac = fld >= T_QX ? faciq : (fld >= PXX ? faci : 1.0);
+        double m1[1][NB], l1[1][NB], K1[1][NK], L1[1][NK];
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            const long o = ((long)r * NQ + fld) * g.N + cell;
+            m1[0][r] = q[o];
+            l1[0][r] = ap.b * lam[o] * fac;
+        }
+        if (side == 0) {
+            fz::face_coeffs<NB, 1, DIR, 1, NK>(m1, K1);
+            lift_coeffs<NB, 1, DIR, 1, NK>(l1, L1);
+        } else {
+            fz::face_coeffs<NB, 1, DIR, 0, NK>(m1, K1);
+            lift_coeffs<NB, 1, DIR, 0, NK>(l1, L1);
+        }
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            ks[side][fld][k][l] = K1[0][k];
+            ls[side][fld][k][l] = L1[0][k];
+        }
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int l = (int)(threadIdx.x & 63);
+    const long fid = f0 + l;
+    if (fid >= nfaces) return;
+    const auto& FP = fz::face_poly<NB>();
+    const double dxi_d = DIR == 0 ? g.dxi : DIR == 1 ? g.dyi : g.dzi;
+    double QL[NQ], QR[NQ], mu[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        double sl = ks[0][f][0][l], sr = ks[1][f][0][l];
+        double tl = ls[0][f][0][l] * FP.wb[p][0], tr = ls[1][f][0][l] * FP.wb[p][0];
+#pragma unroll
+        for (int k = 1; k < NK; k++) {
+            sl += ks[0][f][k][l] * FP.b[p][k];
+            sr += ks[1][f][k][l] * FP.b[p][k];
+            tl += ls[0][f][k][l] * FP.wb[p][k];
+            tr += ls[1][f][k][l] * FP.wb[p][k];
+        }
+        QL[f] = sl;
+        QR[f] = sr;
+        mu[f] = dxi_d * (tl + tr);
+    }
+    double bL[NQ], bR[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) { bL[f] = 0.; bR[f] = 0.; }
+    face_vjp<NQ, RM, DIR>(ph, QL, QR, mu, bL, bR);
+#pragma unroll
+    for (int f = 0; f < NQ; f++) {
+        const long o = ((long)f * NF + p) * nfaces + fid;
+        bql[o] = bL[f];
+        bqr[o] = bR[f];
+    }
+}
+
+// per-block partial of d(lam.out)/dcoll for workgroups of 13 waves
+__device__ __forceinline__ void block_partial13(double v, double* partial) {
+    __shared__ double red[CELL13_WAVES];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.;
+        for (int w = 0; w < CELL13_WAVES; w++) s += red[w];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// One field per wave over 64 consecutive cells (as k_cell13).  The volume
+// transpose takes one row of the rule (QP points) per exchange: each wave
+// writes its field's point value and the cotangents of its three point
+// fluxes (the transposes of k_cell's volume sums); one wave per point then
+// reverses the point's physical fluxes (flux_vjp) and shares the state
+// cotangent, which each wave projects onto its field's modes.  The faces add
+// bfc^T of the cotangents written by k_face13_vjp.  Then the sources, the
+// stage map (lo + dt*rt, the accumulated terms) and the frozen limiter, as
+// cell_vjp_finish; d/dcoll as cell_vjp_parts.
+template <int NB, int QP, int RM>
+__global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13_vjp(
+    const double* __restrict__ q, const double* __restrict__ lam, double* __restrict__ lam_in,
+    const double* __restrict__ bql0, const double* __restrict__ bql1, const double* __restrict__ bql2,
+    const double* __restrict__ bqr0, const double* __restrict__ bqr1, const double* __restrict__ bqr2,
+    const Tables* __restrict__ T, Grid g, Phys ph, AdjParams ap) {
+    constexpr int NQ = 13, NF = QP * QP, NPG = QP * QP * QP, NCH = QP;
+    __shared__ double qp_s[NQ][NCH][64];
+    __shared__ double bf_s[NQ][3][NCH][64];
+    __shared__ double bq_s[NQ][NCH][64];
+    const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const long c = (long)xcd_block() * 64 + lane;
+    const bool live = c < g.N;
+    const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
+    const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));
+    const double dt = *ap.dt;
+    const double faci = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll) : 1.0;
+    const double faciq = (ap.ivis == 1 && !ap.rhs_only) ? 1. / (1. + dt * ap.coll_q) : 1.0;
+    const double fac = f >= T_QX ? faciq : (f >= PXX ? faci : 1.0);
+    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+    double mf[NB], lof[NB], rt[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        const long o = qoff(g, cc, f, r);
+        mf[r] = q[o];
+        lof[r] = ap.b * lam[o] * fac;
+        rt[r] = 0.;
+    }
+    // ---- volume transpose; every wave reaches every barrier ----
+#pragma unroll 1
+    for (int p0 = 0; p0 < NPG; p0 += NCH) {
+#pragma unroll
+        for (int qq = 0; qq < NCH; qq++) {
+            const int p = p0 + qq;
+            const double* b = T->bint[p];
+            double v = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) v += b[r] * mf[r];
+            qp_s[f][qq][lane] = v;
+            // cotangents of F_x, F_y, F_z of field f at point p: the
+            // transposes of k_cell's volume sums acc[*] and intr[*]
+            double bx = T->cb[KX] * lof[KX], by = T->cb[KY] * lof[KY], bz = T->cb[KZ] * lof[KZ];
+            if constexpr (NB > 4) {
+                by += T->cb[KYZ] * b[KZ] * lof[KYZ];
+                bz += T->cb[KYZ] * b[KY] * lof[KYZ];
+                bx += T->cb[KZX] * b[KZ] * lof[KZX];
+                bz += T->cb[KZX] * b[KX] * lof[KZX];
+                bx += T->cb[KXY] * b[KY] * lof[KXY];
+                by += T->cb[KXY] * b[KX] * lof[KXY];
+                if constexpr (NB == 10) {
+                    bx += T->cb[KXX] * 3. * b[KX] * lof[KXX];
+                    by += T->cb[KYY] * 3. * b[KY] * lof[KYY];
+                    bz += T->cb[KZZ] * 3. * b[KZ] * lof[KZZ];
+                } else {
+                    bx += T->cb[KXYZ] * b[KYZ] * lof[KXYZ];
+                    by += T->cb[KXYZ] * b[KZX] * lof[KXYZ];
+                    bz += T->cb[KXYZ] * b[KXY] * lof[KXYZ];
+                }
+            }
+            const double w = 0.25 * T->w3[p];
+            bf_s[f][0][qq][lane] = dxi * w * bx;
+            bf_s[f][1][qq][lane] = dyi * w * by;
+            bf_s[f][2][qq][lane] = dzi * w * bz;
+        }
+        __syncthreads();
+        if (f < NCH) {
+            double Qp[NQ], bx[NQ], by[NQ], bz[NQ], bQ[NQ];
+#pragma unroll
+            for (int e = 0; e < NQ; e++) {
+                Qp[e] = qp_s[e][f][lane];
+                bx[e] = bf_s[e][0][f][lane];
+                by[e] = bf_s[e][1][f][lane];
+                bz[e] = bf_s[e][2][f][lane];
+                bQ[e] = 0.;
+            }
+            const pr::Prim st = pr::prim<NQ>(ph, Qp);
+            flux_vjp<NQ, 0>(ph, Qp, st, bx, bQ);
+            flux_vjp<NQ, 1>(ph, Qp, st, by, bQ);
+            flux_vjp<NQ, 2>(ph, Qp, st, bz, bQ);
+#pragma unroll
+            for (int e = 0; e < NQ; e++) bq_s[e][f][lane] = bQ[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < NCH; qq++) {
+            const double bQf = bq_s[f][qq][lane];
+            const double* b = T->bint[p0 + qq];
+#pragma unroll
+            for (int r = 0; r < NB; r++) rt[r] += b[r] * bQf;
+        }
+        __syncthreads();  // the next row overwrites the exchange arrays
+    }
+    double dcoll = 0.;
+    if (live) {
+        // ---- faces: the cell is the right side of its low faces and the
+        // left side of its high faces ----
+        const long nfl[3] = {(long)(g.nx + 1) * g.ny * g.nz, (long)g.nx * (g.ny + 1) * g.nz,
+                             (long)g.nx * g.ny * (g.nz + 1)};
+        const long fm[3] = {(long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k),
+                            (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k),
+                            (long)i + (long)g.nx * ((long)j + (long)g.ny * k)};
+        const long fp[3] = {fm[0] + 1, fm[1] + g.nx, fm[2] + (long)g.nx * g.ny};
+        const double* const bqr[3] = {bqr0, bqr1, bqr2};
+        const double* const bql[3] = {bql0, bql1, bql2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll 1
+            for (int p = 0; p < NF; p++) {
+                const long o = (long)f * NF + p;
+                const double vR = bqr[d][o * nfl[d] + fm[d]];  // this cell is the right side of its low face
+                const double vL = bql[d][o * nfl[d] + fp[d]];  // and the left side of its high face
+#pragma unroll
+                for (int r = 0; r < NB; r++) rt[r] += T->bfc[2 * d][p][r] * vR + T->bfc[2 * d + 1][p][r] * vL;
+            }
+        }
+        // ---- sources (as cell_vjp_parts) ----
+        if (ap.ivis == 0) {
+            if (f >= PXX && f <= T_PYZ && ap.coll != 0.)
+#pragma unroll
+                for (int r = 0; r < NB; r++) {
+                    rt[r] += -ap.coll * lof[r];
+                    dcoll += -dt * lof[r] * mf[r];
+                }
+            if (f >= T_QX && ap.coll_q != 0.)
+#pragma unroll
+                for (int r = 0; r < NB; r++) rt[r] += -ap.coll_q * lof[r];
+        }
+        if (ap.ivis == 1 && ap.coll != 0. && ap.out && f >= PXX && f <= T_PYZ) {
+            // qn is read only when its weight a is nonzero (callers pass no
+            // qn for the first stage)
+            const bool with_qn = ap.a != 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+                const long o = qoff(g, c, f, r);
+                const double bv = with_qn ? ap.out[o] - ap.a * ap.qn[o] : ap.out[o];
+                dcoll += -dt * faci * lam[o] * bv;
+            }
+        }
+        if (ap.bf[0] != 0. || ap.bf[1] != 0. || ap.bf[2] != 0.) {
+            // lo of the Euler fields (relaxation factor 1)
+            if (f == RH)
+#pragma unroll
+                for (int r = 0; r < NB; r++)
+                    rt[r] += ap.bf[0] * ap.b * lam[qoff(g, c, MX, r)] + ap.bf[1] * ap.b * lam[qoff(g, c, MY, r)] +
+                             ap.bf[2] * ap.b * lam[qoff(g, c, MZ, r)];
+            if (f >= MX && f <= MZ)
+#pragma unroll
+                for (int r = 0; r < NB; r++) rt[r] += ap.bf[f - MX] * ap.b * lam[qoff(g, c, EN, r)];
+        }
+        // ---- finish (as cell_vjp_finish) ----
+        if (ap.rhs_only) {
+#pragma unroll
+            for (int r = 0; r < NB; r++) lam_in[qoff(g, c, f, r)] = rt[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < NB; r++) rt[r] = lof[r] + dt * rt[r];
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+                if (ap.acc[t])
+#pragma unroll
+                    for (int r = 0; r < NB; r++) rt[r] += ap.w[t] * ap.acc[t][qoff(g, c, f, r)];
+            if (ap.q_raw && f <= EN) {
+                double raw[5][NB];
+#pragma unroll
+                for (int e = 0; e < 5; e++)
+#pragma unroll
+                    for (int r = 0; r < NB; r++) raw[e][r] = ap.q_raw[qoff(g, c, e, r)];
+                const LimFactors F = limit_factors<NB, QP, 5>(raw, ap.lp);
+                if (F.zero) {
+#pragma unroll
+                    for (int r = 1; r < NB; r++) rt[r] = 0.0;
+                    if (F.floor && f == RH) rt[0] = 0.0;
+                } else {
+                    const double th = f == RH ? F.th_rho * F.th_p : F.th_p;
+#pragma unroll
+                    for (int r = 1; r < NB; r++) rt[r] *= th;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < NB; r++) lam_in[qoff(g, c, f, r)] = rt[r];
+        }
+    }
+    if (ap.partial) block_partial13(dcoll, ap.partial);
+}
+
 // adds the block partials to *acc (one block, fixed order)
 static __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, int n,
                                                              double* __restrict__ acc) {

@@ -314,6 +314,7 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
     if (const char* ro = std::getenv("HD_STAGE_ROWS")) c->rows_zfast = std::strcmp(ro, "yz") != 0;
     if (const char* lm = std::getenv("HD_LIMITER")) c->table_limiter = std::strcmp(lm, "table") == 0;
     if (const char* fc = std::getenv("HD_FACES")) c->table_faces = std::strcmp(fc, "table") == 0;
+    if (const char* vj = std::getenv("HD_VJP")) c->fused_vjp = std::strcmp(vj, "fused") == 0;
     // workgroup size of the fused stage kernel: 64, 128 or 256 threads
     if (const char* bs = std::getenv("HD_STAGE_BLOCK")) {
         const int v = std::atoi(bs);
@@ -335,7 +336,8 @@ int hd_destroy(hd_ctx* c) {
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {c->dtab,     c->fl[0],    c->fl[1],     c->fl[2],      c->partial,    c->err,
                     c->bad,      c->scratch_dt, c->stage_buf, c->step_ctr, c->noise_f[0], c->noise_f[1],
-                    c->noise_f[2], c->noise_v,  c->vjp_partial, c->inflow, c->mask};
+                    c->noise_f[2], c->noise_v,  c->vjp_partial, c->inflow, c->mask,
+                    c->bq2[0],     c->bq2[1],   c->bq2[2]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -492,6 +494,23 @@ int hd_apply_mask(hd_ctx* c, double* q, void* stream) {
     return apply_mask(c, q, (hipStream_t)stream);
 }
 
+// The split 13-moment transpose (periodic grids) writes the left cells' face
+// cotangents into fl and the right cells' into bq2, allocated here once.
+static int ensure_vjp13_buffers(hd_ctx* c) {
+    if (c->cfg.nQ != 13 || c->fused_vjp || c->bq2[0]) return HD_OK;
+    for (int b = 0; b < 6; b++)
+        if (c->cfg.bc[b] != HD_BC_PERIODIC) return HD_OK;
+    const Grid& g = c->grid;
+    const size_t nf = (size_t)c->nface * c->nQ;
+    const size_t nfd[3] = {(size_t)(g.nx + 1) * g.ny * g.nz, (size_t)g.nx * (g.ny + 1) * g.nz,
+                           (size_t)g.nx * g.ny * (g.nz + 1)};
+    for (int d = 0; d < 3; d++) {
+        const int r = alloc(c, (void**)&c->bq2[d], nf * nfd[d] * sizeof(double));
+        if (r) return r;
+    }
+    return HD_OK;
+}
+
 int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* lam_in, const double* dt_dev,
                  const hd_stage_vjp_args* args, void* stream) {
     if (!c || !q_in || !lam_out || !lam_in || !dt_dev || !args) return fail(HD_ERR_INVALID, "NULL argument");
@@ -524,6 +543,7 @@ int hd_stage_vjp(hd_ctx* c, const double* q_in, const double* lam_out, double* l
     ap.rhs_only = 0;
     ap.inflow = c->inflow;
     hipStream_t s = (hipStream_t)stream;
+    if (int r = ensure_vjp13_buffers(c)) return r;
     o.stage_vjp(c, q_in, lam_out, lam_in, ap, s);
     HIPCHK(hipGetLastError());
     if (args->dcoll_dev) {
@@ -556,6 +576,7 @@ int hd_rhs_vjp(hd_ctx* c, const double* q, const double* lam, double* out, void*
     ap.lp = c->lp;
     ap.rhs_only = 1;
     ap.inflow = c->inflow;
+    if (int r = ensure_vjp13_buffers(c)) return r;
     o.stage_vjp(c, q, lam, out, ap, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return HD_OK;

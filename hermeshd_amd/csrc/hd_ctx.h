@@ -27,6 +27,9 @@ struct hd_ctx {
     hd::Tables htab;
     hd::Tables* dtab = nullptr;
     double* fl[3] = {nullptr, nullptr, nullptr};  // face fluxes per direction (table-driven kernels)
+    // periodic 13-moment transpose: the right cells' face cotangents (the left
+    // cells' reuse fl); allocated at the first such transpose
+    double* bq2[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;                    // wave-speed partial maxima
     int npart = 0;
     int* err = nullptr;
@@ -64,6 +67,8 @@ struct hd_ctx {
     int table_limiter = 0;
     // one-thread-per-face k_face_flux for periodic 13-moment grids too (HD_FACES=table)
     int table_faces = 0;
+    // fused per-cell transpose for periodic 13-moment grids too (HD_VJP=fused)
+    int fused_vjp = 0;
     int64_t bytes = 0;
     int device = 0;
     // stage timing (hd_timing_enable): event pairs recorded around each stage

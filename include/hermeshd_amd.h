@@ -153,7 +153,8 @@ typedef struct hd_stage_vjp_args {
     const double *acc0, *acc1; /* NULL or arrays added to lam_in with weights w0, w1 */
     double w0, w1;
     const double *q_raw;       /* stage input before the limiter (used when the limiter is on) */
-    const double *out, *qn;    /* forward stage output and q_n, for d/dcoll (out NULL: skip) */
+    const double *out, *qn;    /* forward stage output and q_n, for d/dcoll (out NULL: skip;
+                                  qn is read only when a != 0, and then required) */
     double *dcoll_dev;         /* device double: += d(lam_out . out)/dcoll (NULL: skip) */
 } hd_stage_vjp_args;
 
