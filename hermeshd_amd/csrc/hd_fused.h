@@ -774,14 +774,15 @@ __global__ __launch_bounds__(256) void k_stage_xs(const double* __restrict__ qin
 }
 
 // Face fluxes of the 13-moment model on a grid periodic in every direction,
-// in k_face_flux's output layout fl[((f*NF + p)*nfaces) + face].  A workgroup
+// as face moments fl[((f*NK + k)*nfaces) + face] (lift13_mom).  A workgroup
 // owns 64 consecutive faces of direction DIR, one wave per face point, so no
 // lane holds more than one point's states.  Phase 1 turns the modes of the
 // cells on both sides into face-polynomial coefficients (face_coeffs, NK per
 // field and side) in LDS, one (side, field, face) item per thread.  Phase 2
 // evaluates point p of every face on wave p (face_eval), then the physical
-// fluxes and the Riemann solver of k_face_flux.  Lanes past the last face
-// reuse the last face in phase 1 and store nothing.
+// fluxes and the Riemann solver of k_face_flux.  Phase 3 reduces the points
+// of each (field, face) to the NK face moments.  Lanes past the last face
+// reuse the last face and store nothing.
 template <int NB, int QP, int RM, int DIR>
 __global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restrict__ q, double* __restrict__ fl,
                                                         Grid g, Phys ph) {
@@ -809,10 +810,9 @@ __global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restric
         for (int k = 0; k < NK; k++) ks[side][fld][k][l] = K1[0][k];
     }
     __syncthreads();
+    // lanes past the last face evaluate the last face again and store nothing
     const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int l = (int)(threadIdx.x & 63);
-    const long fid = f0 + l;
-    if (fid >= nfaces) return;
     const auto& FP = face_poly<NB>();
     double QL[NQ], QR[NQ];
 #pragma unroll
@@ -833,9 +833,231 @@ __global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restric
     for (int f = 0; f < NQ; f++) out[f] = 0.5 * (FL[f] + FR[f]);
     if constexpr (RM == RIEMANN_HLLC) hllc_point<DIR>(ph, QL, QR, FL, FR, out);  // ideal gas: no last-state speed
     if constexpr (RM == RIEMANN_ROE) roe_point<DIR, NQ>(ph, QL, QR, FL, FR, out);
+    // phase 3: the face moments M_k = sum_p wb_p[k] F_p of each field, the
+    // point fluxes exchanged through ks once every wave has read it
+    static_assert(NF <= 2 * NK, "the point fluxes fit in ks");
+    double* vs = &ks[0][0][0][0];
+    __syncthreads();
 #pragma unroll
-    for (int f = 0; f < NQ; f++) fl[((long)f * NF + p) * nfaces + fid] = out[f];
+    for (int f = 0; f < NQ; f++) vs[(f * NF + p) * 64 + l] = out[f];
+    __syncthreads();
+    for (int it = (int)threadIdx.x; it < NQ * NK * 64; it += NF * 64) {
+        const int lf = it & 63;
+        const int fk = __builtin_amdgcn_readfirstlane(it >> 6);
+        const int f = fk / NK, k = fk - f * NK;
+        double s = 0.;
+#pragma unroll
+        for (int pp = 0; pp < NF; pp++) s += FP.wb[pp][k] * vs[(f * NF + pp) * 64 + lf];
+        if (f0 + lf < nfaces) fl[((long)f * NK + k) * nfaces + f0 + lf] = s;
+    }
+}
+
+// Face lift of one field in direction D from the face moments of k_face13:
+// sum_p w2_p phi_r(p) F_p on a face is Lface(deg_D(r), side) M_{k(r)}, with
+// k(r) the mode's face-polynomial pair, so the sums of lift13_dir become
+// t[r] += di (M+_{k(r)} - Lface(deg_D(r), 0) M-_{k(r)}), M+ of the cell's
+// high face and M- of its low face.
+template <int NB, int D>
+HD_DEV constexpr int lift_k(int r) {
+    return pair_index(deg<NB>(r, row_axis(D)), deg<NB>(r, col_axis(D)));
+}
+template <int NB, int D>
+__device__ __forceinline__ void lift13_mom(const double* __restrict__ fl, long nfl, long fm, long fp, double di,
+                                           int f, double (&t)[NB]) {
+    constexpr int NK = face_nk<NB>();
+    double mm[NK], mp[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        mm[k] = fl[((long)f * NK + k) * nfl + fm];
+        mp[k] = fl[((long)f * NK + k) * nfl + fp];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        const int k = lift_k<NB, D>(r);
+        const double s = deg<NB>(r, D) == 1 ? mp[k] + mm[k] : mp[k] - mm[k];
+        t[r] = D == 0 ? di * s : t[r] + di * s;
+    }
 }
 
 }  // namespace fz
+
+
+// MOM: the face arrays hold the face moments of fz::k_face13 (periodic
+// grids); otherwise the point fluxes of k_face_flux.
+template <int NB, int QP, bool MOM>
+__global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
+    const double* __restrict__ qin, const double* __restrict__ qn, double* __restrict__ qout,
+    const double* __restrict__ flx, const double* __restrict__ fly, const double* __restrict__ flz,
+    const Tables* __restrict__ T, Grid g, Phys ph, StageParams sp) {
+    constexpr int NQ = 13, NF = QP * QP, NPG = QP * QP * QP;
+    constexpr int NS = (NB > 4) ? 12 : 3;
+    constexpr int NCH = NF;  // volume points per exchange: one plane of the 3D rule
+    static_assert(NCH <= CELL13_WAVES, "one wave per point computes the primitives");
+    __shared__ double qp_s[NQ][NCH][64];
+    __shared__ double pr_s[NCH][NPRIM13][64];
+    const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const long c = (long)xcd_block() * 64 + lane;  // XCD-contiguous ranges of cells
+    const bool live = c < g.N;
+    const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
+    const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));
+    double mf[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) mf[r] = qin[qoff(g, cc, f, r)];
+    // ---- volume integral: every wave reaches every barrier ----
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) acc[s] = 0.;
+    // volume integral, NCH points per exchange: each wave writes its field's
+    // point values, then reads the 13 values of each point; every wave
+    // reaches every barrier
+#pragma unroll 1
+    for (int p0 = 0; p0 < NPG; p0 += NCH) {
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            const double* b = T->bint[p0 + q];
+            double v = 0.;
+#pragma unroll
+            for (int r = 0; r < NB; r++) v += b[r] * mf[r];
+            qp_s[f][q][lane] = v;
+        }
+        __syncthreads();
+        // the primitives of point f of the plane, once per point
+        if (f < NCH) {
+            double Qp[NQ], pr[NPRIM13];
+#pragma unroll
+            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][f][lane];
+            prim13(ph, Qp, pr);
+#pragma unroll
+            for (int e = 0; e < NPRIM13; e++) pr_s[f][e][lane] = pr[e];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < NCH; q++) {
+            const int p = p0 + q;
+            const double* b = T->bint[p];
+            double Qp[NQ], pr[NPRIM13];
+#pragma unroll
+            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][q][lane];
+#pragma unroll
+            for (int e = 0; e < NPRIM13; e++) pr[e] = pr_s[q][e][lane];
+            double Fx, Fy, Fz;
+            flux13_field(ph, Qp, pr, f, Fx, Fy, Fz);
+            const double w = T->w3[p];
+            acc[0] += w * Fx;
+            acc[1] += w * Fy;
+            acc[2] += w * Fz;
+            if constexpr (NB > 4) {
+                acc[3] += w * b[KZ] * Fy;
+                acc[4] += w * b[KY] * Fz;
+                acc[5] += w * b[KZ] * Fx;
+                acc[6] += w * b[KX] * Fz;
+                acc[7] += w * b[KY] * Fx;
+                acc[8] += w * b[KX] * Fy;
+                if constexpr (NB == 10) {
+                    acc[9] += w * 3. * b[KX] * Fx;
+                    acc[10] += w * 3. * b[KY] * Fy;
+                    acc[11] += w * 3. * b[KZ] * Fz;
+                } else {
+                    acc[9] += w * b[KYZ] * Fx;
+                    acc[10] += w * b[KZX] * Fy;
+                    acc[11] += w * b[KXY] * Fz;
+                }
+            }
+        }
+        __syncthreads();  // the next plane overwrites qp_s and pr_s
+    }
+    if (!live) return;  // no barrier below
+    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+    double intr[NB];
+    intr[0] = 0.;
+    intr[KX] = 0.25 * T->cb[KX] * dxi * acc[0];
+    intr[KY] = 0.25 * T->cb[KY] * dyi * acc[1];
+    intr[KZ] = 0.25 * T->cb[KZ] * dzi * acc[2];
+    if constexpr (NB > 4) {
+        intr[KYZ] = 0.25 * T->cb[KYZ] * (dyi * acc[3] + dzi * acc[4]);
+        intr[KZX] = 0.25 * T->cb[KZX] * (dxi * acc[5] + dzi * acc[6]);
+        intr[KXY] = 0.25 * T->cb[KXY] * (dxi * acc[7] + dyi * acc[8]);
+        if constexpr (NB == 10) {
+            intr[KXX] = 0.25 * T->cb[KXX] * (dxi * acc[9]);
+            intr[KYY] = 0.25 * T->cb[KYY] * (dyi * acc[10]);
+            intr[KZZ] = 0.25 * T->cb[KZZ] * (dzi * acc[11]);
+        } else {
+            intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[9] + dyi * acc[10] + dzi * acc[11]);
+        }
+    }
+    // face lift of field f, one direction at a time:
+    // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
+    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
+    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
+    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+    const long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz,
+               nfz = (long)g.nx * g.ny * (g.nz + 1);
+    double t[NB];
+    if constexpr (MOM) {
+        fz::lift13_mom<NB, 0>(flx, nfx, fxm, fxm + 1, dxi, f, t);
+        fz::lift13_mom<NB, 1>(fly, nfy, fym, fym + g.nx, dyi, f, t);
+        fz::lift13_mom<NB, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, t);
+    } else {
+        lift13_dir<NB, NF, 0>(flx, nfx, fxm, fxm + 1, dxi, f, T, t);
+        lift13_dir<NB, NF, 1>(fly, nfy, fym, fym + g.nx, dyi, f, T, t);
+        lift13_dir<NB, NF, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
+    }
+    double glf[NB];
+    glf[0] = 0.25 * t[0];
+#pragma unroll
+    for (int r = 1; r < NB; r++) glf[r] = 0.25 * T->cb[r] * t[r] - intr[r];
+    // ---- source: explicit relaxation (ivis 0), body force ----
+    double src[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) src[r] = 0.;
+    const double rate = f >= T_QX ? sp.coll_q : sp.coll;
+    if (sp.ivis == 0 && f >= T_PXX && rate != 0.) {
+#pragma unroll 1
+        for (int r = 0; r < NB; r++) {
+            double s = 0.;
+#pragma unroll 1
+            for (int p = 0; p < NPG; p++) {
+                double qv = 0.;
+#pragma unroll
+                for (int rr = 0; rr < NB; rr++) qv += T->bint[p][rr] * mf[rr];
+                s += T->bwint[p][r] * (-rate * qv);
+            }
+            src[r] = 0.125 * T->cb[r] * s;
+        }
+    }
+    if (f >= MX && f <= MZ && sp.bf[f - MX] != 0.)
+#pragma unroll
+        for (int r = 0; r < NB; r++) src[r] += sp.bf[f - MX] * qin[qoff(g, c, RH, r)];
+    if (f == EN && (sp.bf[0] != 0. || sp.bf[1] != 0. || sp.bf[2] != 0.))
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+            src[r] += sp.bf[0] * qin[qoff(g, c, MX, r)] + sp.bf[1] * qin[qoff(g, c, MY, r)] +
+                      sp.bf[2] * qin[qoff(g, c, MZ, r)];
+    // ---- update, SSP combination, NaN check of mode 1 ----
+    if (sp.mode == 1) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) qout[qoff(g, c, f, r)] = -(glf[r] - src[r]);
+        return;
+    }
+    const double dt = *sp.dt;
+    const double fac = f >= T_QX ? 1. / (1. + dt * sp.coll_q) : 1. / (1. + dt * sp.coll);
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        double v = mf[r] - dt * (glf[r] - src[r]);
+        if (sp.ivis == 1 && f >= T_PXX) v = fac * v;
+        const long o = qoff(g, c, f, r);
+        if (r == 0 && v != v) bad = true;
+        if (sp.combine == 1) v = 0.5 * (qn[o] + v);
+        else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
+        else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
+        qout[o] = v;
+    }
+    if (bad) {
+        atomicOr(sp.err, 1);
+        atomicMin(sp.bad, (unsigned long long)c);
+    }
+}
+
 }  // namespace hd

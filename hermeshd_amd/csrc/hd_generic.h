@@ -625,174 +625,6 @@ __device__ __forceinline__ void lift13_dir(const double* __restrict__ fl, long n
 
 constexpr int CELL13_WAVES = 13;
 
-template <int NB, int QP>
-__global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
-    const double* __restrict__ qin, const double* __restrict__ qn, double* __restrict__ qout,
-    const double* __restrict__ flx, const double* __restrict__ fly, const double* __restrict__ flz,
-    const Tables* __restrict__ T, Grid g, Phys ph, StageParams sp) {
-    constexpr int NQ = 13, NF = QP * QP, NPG = QP * QP * QP;
-    constexpr int NS = (NB > 4) ? 12 : 3;
-    constexpr int NCH = NF;  // volume points per exchange: one plane of the 3D rule
-    static_assert(NCH <= CELL13_WAVES, "one wave per point computes the primitives");
-    __shared__ double qp_s[NQ][NCH][64];
-    __shared__ double pr_s[NCH][NPRIM13][64];
-    const int f = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lane = (int)(threadIdx.x & 63);
-    const long c = (long)xcd_block() * 64 + lane;  // XCD-contiguous ranges of cells
-    const bool live = c < g.N;
-    const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
-    const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));
-    double mf[NB];
-#pragma unroll
-    for (int r = 0; r < NB; r++) mf[r] = qin[qoff(g, cc, f, r)];
-    // ---- volume integral: every wave reaches every barrier ----
-    double acc[NS];
-#pragma unroll
-    for (int s = 0; s < NS; s++) acc[s] = 0.;
-    // volume integral, NCH points per exchange: each wave writes its field's
-    // point values, then reads the 13 values of each point; every wave
-    // reaches every barrier
-#pragma unroll 1
-    for (int p0 = 0; p0 < NPG; p0 += NCH) {
-#pragma unroll
-        for (int q = 0; q < NCH; q++) {
-            const double* b = T->bint[p0 + q];
-            double v = 0.;
-#pragma unroll
-            for (int r = 0; r < NB; r++) v += b[r] * mf[r];
-            qp_s[f][q][lane] = v;
-        }
-        __syncthreads();
-        // the primitives of point f of the plane, once per point
-        if (f < NCH) {
-            double Qp[NQ], pr[NPRIM13];
-#pragma unroll
-            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][f][lane];
-            prim13(ph, Qp, pr);
-#pragma unroll
-            for (int e = 0; e < NPRIM13; e++) pr_s[f][e][lane] = pr[e];
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < NCH; q++) {
-            const int p = p0 + q;
-            const double* b = T->bint[p];
-            double Qp[NQ], pr[NPRIM13];
-#pragma unroll
-            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][q][lane];
-#pragma unroll
-            for (int e = 0; e < NPRIM13; e++) pr[e] = pr_s[q][e][lane];
-            double Fx, Fy, Fz;
-            flux13_field(ph, Qp, pr, f, Fx, Fy, Fz);
-            const double w = T->w3[p];
-            acc[0] += w * Fx;
-            acc[1] += w * Fy;
-            acc[2] += w * Fz;
-            if constexpr (NB > 4) {
-                acc[3] += w * b[KZ] * Fy;
-                acc[4] += w * b[KY] * Fz;
-                acc[5] += w * b[KZ] * Fx;
-                acc[6] += w * b[KX] * Fz;
-                acc[7] += w * b[KY] * Fx;
-                acc[8] += w * b[KX] * Fy;
-                if constexpr (NB == 10) {
-                    acc[9] += w * 3. * b[KX] * Fx;
-                    acc[10] += w * 3. * b[KY] * Fy;
-                    acc[11] += w * 3. * b[KZ] * Fz;
-                } else {
-                    acc[9] += w * b[KYZ] * Fx;
-                    acc[10] += w * b[KZX] * Fy;
-                    acc[11] += w * b[KXY] * Fz;
-                }
-            }
-        }
-        __syncthreads();  // the next plane overwrites qp_s and pr_s
-    }
-    if (!live) return;  // no barrier below
-    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
-    double intr[NB];
-    intr[0] = 0.;
-    intr[KX] = 0.25 * T->cb[KX] * dxi * acc[0];
-    intr[KY] = 0.25 * T->cb[KY] * dyi * acc[1];
-    intr[KZ] = 0.25 * T->cb[KZ] * dzi * acc[2];
-    if constexpr (NB > 4) {
-        intr[KYZ] = 0.25 * T->cb[KYZ] * (dyi * acc[3] + dzi * acc[4]);
-        intr[KZX] = 0.25 * T->cb[KZX] * (dxi * acc[5] + dzi * acc[6]);
-        intr[KXY] = 0.25 * T->cb[KXY] * (dxi * acc[7] + dyi * acc[8]);
-        if constexpr (NB == 10) {
-            intr[KXX] = 0.25 * T->cb[KXX] * (dxi * acc[9]);
-            intr[KYY] = 0.25 * T->cb[KYY] * (dyi * acc[10]);
-            intr[KZZ] = 0.25 * T->cb[KZZ] * (dzi * acc[11]);
-        } else {
-            intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[9] + dyi * acc[10] + dzi * acc[11]);
-        }
-    }
-    // face lift of field f, one direction at a time:
-    // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
-    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
-    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
-    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
-    double t[NB];
-    lift13_dir<NB, NF, 0>(flx, (long)(g.nx + 1) * g.ny * g.nz, fxm, fxm + 1, dxi, f, T, t);
-    lift13_dir<NB, NF, 1>(fly, (long)g.nx * (g.ny + 1) * g.nz, fym, fym + g.nx, dyi, f, T, t);
-    lift13_dir<NB, NF, 2>(flz, (long)g.nx * g.ny * (g.nz + 1), fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
-    double glf[NB];
-    glf[0] = 0.25 * t[0];
-#pragma unroll
-    for (int r = 1; r < NB; r++) glf[r] = 0.25 * T->cb[r] * t[r] - intr[r];
-    // ---- source: explicit relaxation (ivis 0), body force ----
-    double src[NB];
-#pragma unroll
-    for (int r = 0; r < NB; r++) src[r] = 0.;
-    const double rate = f >= T_QX ? sp.coll_q : sp.coll;
-    if (sp.ivis == 0 && f >= T_PXX && rate != 0.) {
-#pragma unroll 1
-        for (int r = 0; r < NB; r++) {
-            double s = 0.;
-#pragma unroll 1
-            for (int p = 0; p < NPG; p++) {
-                double qv = 0.;
-#pragma unroll
-                for (int rr = 0; rr < NB; rr++) qv += T->bint[p][rr] * mf[rr];
-                s += T->bwint[p][r] * (-rate * qv);
-            }
-            src[r] = 0.125 * T->cb[r] * s;
-        }
-    }
-    if (f >= MX && f <= MZ && sp.bf[f - MX] != 0.)
-#pragma unroll
-        for (int r = 0; r < NB; r++) src[r] += sp.bf[f - MX] * qin[qoff(g, c, RH, r)];
-    if (f == EN && (sp.bf[0] != 0. || sp.bf[1] != 0. || sp.bf[2] != 0.))
-#pragma unroll
-        for (int r = 0; r < NB; r++)
-            src[r] += sp.bf[0] * qin[qoff(g, c, MX, r)] + sp.bf[1] * qin[qoff(g, c, MY, r)] +
-                      sp.bf[2] * qin[qoff(g, c, MZ, r)];
-    // ---- update, SSP combination, NaN check of mode 1 ----
-    if (sp.mode == 1) {
-#pragma unroll
-        for (int r = 0; r < NB; r++) qout[qoff(g, c, f, r)] = -(glf[r] - src[r]);
-        return;
-    }
-    const double dt = *sp.dt;
-    const double fac = f >= T_QX ? 1. / (1. + dt * sp.coll_q) : 1. / (1. + dt * sp.coll);
-    bool bad = false;
-#pragma unroll
-    for (int r = 0; r < NB; r++) {
-        double v = mf[r] - dt * (glf[r] - src[r]);
-        if (sp.ivis == 1 && f >= T_PXX) v = fac * v;
-        const long o = qoff(g, c, f, r);
-        if (r == 0 && v != v) bad = true;
-        if (sp.combine == 1) v = 0.5 * (qn[o] + v);
-        else if (sp.combine == 2) v = 0.75 * qn[o] + 0.25 * v;
-        else if (sp.combine == 3) v = (1. / 3.) * qn[o] + (2. / 3.) * v;
-        qout[o] = v;
-    }
-    if (bad) {
-        atomicOr(sp.err, 1);
-        atomicMin(sp.bad, (unsigned long long)c);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // get_min_dt (src/hermeshd.f90:145-198): per-block max of the wave speed over
 // cells with rho > rh_mult*rh_floor, then a single-block finish that adds the

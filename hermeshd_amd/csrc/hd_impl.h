@@ -37,9 +37,16 @@ struct Impl {
     static void cell(hd_ctx* c, const double* qin, const double* qn, double* qout, const StageParams& sp, hipStream_t s) {
         if constexpr (NQ == 13) {
             // one field per wave, 64 cells per workgroup (every wave of a
-            // workgroup reaches its barriers: the grid is padded, not clipped)
-            hipLaunchKernelGGL((k_cell13<NB, QP>), dim3(blocks_for(c->N, 64)), dim3(CELL13_WAVES * 64), 0, s, qin,
-                               qn, qout, c->fl[0], c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
+            // workgroup reaches its barriers: the grid is padded, not clipped);
+            // face moments from k_face13 on all-periodic grids (as faces())
+            bool periodic = !c->table_faces;
+            for (int b = 0; b < 6; b++) periodic = periodic && c->grid.bc[b] == BC_PERIODIC;
+            if (periodic)
+                hipLaunchKernelGGL((k_cell13<NB, QP, true>), dim3(blocks_for(c->N, 64)), dim3(CELL13_WAVES * 64), 0,
+                                   s, qin, qn, qout, c->fl[0], c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
+            else
+                hipLaunchKernelGGL((k_cell13<NB, QP, false>), dim3(blocks_for(c->N, 64)), dim3(CELL13_WAVES * 64), 0,
+                                   s, qin, qn, qout, c->fl[0], c->fl[1], c->fl[2], c->dtab, c->grid, c->ph, sp);
             return;
         }
         hipLaunchKernelGGL((k_cell<NB, QP, NQ>), dim3(blocks_for(c->N, 128)), dim3(128), 0, s, qin, qn, qout, c->fl[0],
