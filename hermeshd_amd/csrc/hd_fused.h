@@ -285,12 +285,21 @@ struct CellPos {
     long c;
 };
 
+// The planes are reached by stepping one pointer a plane at a time; the
+// empty asm hides the step count from the compiler, which would otherwise
+// keep all NB*NQ plane offsets live in scalar registers (and spill them).
 template <int NB, int NQ>
 __device__ __forceinline__ void load_modes(const double* __restrict__ q, long N, long c, double (&m)[NQ][NB]) {
+    // a global (address space 1) pointer, so the loads stay global loads
+    const __attribute__((address_space(1))) double* p = (const __attribute__((address_space(1))) double*)(q + c);
 #pragma unroll
     for (int r = 0; r < NB; r++)
 #pragma unroll
-        for (int f = 0; f < NQ; f++) m[f][r] = q[((long)r * NQ + f) * N + c];
+        for (int f = 0; f < NQ; f++) {
+            m[f][r] = *p;
+            p += N;
+            asm volatile("" : "+v"(p));
+        }
 }
 template <int NB, int NQ>
 __device__ __forceinline__ void load_modes(const double* __restrict__ q, const Grid& g, long c, double (&m)[NQ][NB]) {
