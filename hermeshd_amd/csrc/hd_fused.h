@@ -945,13 +945,8 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
         for (int q = 0; q < NCH; q++) {
             const int p = p0 + q;
             const double* b = T->bint[p];
-            double Qp[NQ], pr[NPRIM13];
-#pragma unroll
-            for (int e = 0; e < NQ; e++) Qp[e] = qp_s[e][q][lane];
-#pragma unroll
-            for (int e = 0; e < NPRIM13; e++) pr[e] = pr_s[q][e][lane];
             double Fx, Fy, Fz;
-            flux13_field(ph, Qp, pr, f, Fx, Fy, Fz);
+            flux13_field<NCH * 64>(ph, &qp_s[0][q][lane], &pr_s[q][0][lane], f, Fx, Fy, Fz);
             const double w = T->w3[p];
             acc[0] += w * Fx;
             acc[1] += w * Fy;

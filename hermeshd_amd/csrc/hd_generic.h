@@ -562,33 +562,56 @@ __device__ __forceinline__ void prim13(const Phys& ph, const double* Q, double* 
     pr[5] = ph.colq * (P * dni);
 }
 
-// x, y and z flux of field f at one point from its state and primitives
-// (flux_point13, one field at a time; f is uniform over the wave)
-__device__ __forceinline__ void flux13_field(const Phys& ph, const double* Q, const double* pr, int f, double& Fx,
+// x, y and z flux of field f at one point (flux_point13, one field at a
+// time; f is uniform over the wave).  The point's state and primitives are
+// read from the exchange arrays of k_cell13 inside each case, so a wave loads
+// only what its field's flux uses: qc points at field 0 of the point (fields
+// qs apart), pc at primitive 0 (dni, vx, vy, vz, P, col_q*T; 64 apart).
+template <int QS>
+__device__ __forceinline__ void flux13_field(const Phys& ph, const double* qc, const double* pc, int f, double& Fx,
                                              double& Fy, double& Fz) {
-    const double M_x = Q[MX], M_y = Q[MY], M_z = Q[MZ], Ener = Q[EN];
-    const double T_xx = Q[T_PXX], T_yy = Q[T_PYY], T_zz = -(T_xx + T_yy);
-    const double T_xy = Q[T_PXY], T_xz = Q[T_PXZ], T_yz = Q[T_PYZ];
-    const double vx = pr[1], vy = pr[2], vz = pr[3], P = pr[4], ct = pr[5];
+    auto Q = [&](int e) { return qc[e * QS]; };
+    auto V = [&](int e) { return pc[e * 64]; };
     switch (f) {
-        case RH: Fx = M_x; Fy = M_y; Fz = M_z; break;
-        case MX: Fx = M_x * vx + P + T_xx; Fy = M_x * vy + T_xy; Fz = M_x * vz + T_xz; break;
-        case MY: Fx = M_y * vx + T_xy; Fy = M_y * vy + P + T_yy; Fz = M_y * vz + T_yz; break;
-        case MZ: Fx = M_z * vx + T_xz; Fy = M_z * vy + T_yz; Fz = M_z * vz + P + T_zz; break;
-        case EN:
-            Fx = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz + Q[T_QX];
-            Fy = (Ener + P) * vy + T_yy * vy + T_xy * vx + T_yz * vz + Q[T_QY];
-            Fz = (Ener + P) * vz + T_zz * vz + T_xz * vx + T_yz * vy + Q[T_QZ];
+        case RH: Fx = Q(MX); Fy = Q(MY); Fz = Q(MZ); break;
+        case MX: {
+            const double M_x = Q(MX), vx = V(1), vy = V(2), vz = V(3), P = V(4);
+            Fx = M_x * vx + P + Q(T_PXX); Fy = M_x * vy + Q(T_PXY); Fz = M_x * vz + Q(T_PXZ);
             break;
-        case T_PXX: Fx = ph.c4d3cv * vx; Fy = -ph.c2d3cv * vy; Fz = -ph.c2d3cv * vz; break;
-        case T_PYY: Fx = -ph.c2d3cv * vx; Fy = ph.c4d3cv * vy; Fz = -ph.c2d3cv * vz; break;
-        case T_PXY: Fx = ph.colvis * vy; Fy = ph.colvis * vx; Fz = 0.; break;
-        case T_PXZ: Fx = ph.colvis * vz; Fy = 0.; Fz = ph.colvis * vx; break;
-        case T_PYZ: Fx = 0.; Fy = ph.colvis * vz; Fz = ph.colvis * vy; break;
-        default:
+        }
+        case MY: {
+            const double M_y = Q(MY), vx = V(1), vy = V(2), vz = V(3), P = V(4);
+            Fx = M_y * vx + Q(T_PXY); Fy = M_y * vy + P + Q(T_PYY); Fz = M_y * vz + Q(T_PYZ);
+            break;
+        }
+        case MZ: {
+            const double M_z = Q(MZ), vx = V(1), vy = V(2), vz = V(3), P = V(4);
+            const double T_zz = -(Q(T_PXX) + Q(T_PYY));
+            Fx = M_z * vx + Q(T_PXZ); Fy = M_z * vy + Q(T_PYZ); Fz = M_z * vz + P + T_zz;
+            break;
+        }
+        case EN: {
+            const double Ener = Q(EN), vx = V(1), vy = V(2), vz = V(3), P = V(4);
+            const double T_xx = Q(T_PXX), T_yy = Q(T_PYY), T_zz = -(T_xx + T_yy);
+            const double T_xy = Q(T_PXY), T_xz = Q(T_PXZ), T_yz = Q(T_PYZ);
+            Fx = (Ener + P) * vx + T_xx * vx + T_xy * vy + T_xz * vz + Q(T_QX);
+            Fy = (Ener + P) * vy + T_yy * vy + T_xy * vx + T_yz * vz + Q(T_QY);
+            Fz = (Ener + P) * vz + T_zz * vz + T_xz * vx + T_yz * vy + Q(T_QZ);
+            break;
+        }
+        case T_PXX: { const double vx = V(1), vy = V(2), vz = V(3);
+                      Fx = ph.c4d3cv * vx; Fy = -ph.c2d3cv * vy; Fz = -ph.c2d3cv * vz; break; }
+        case T_PYY: { const double vx = V(1), vy = V(2), vz = V(3);
+                      Fx = -ph.c2d3cv * vx; Fy = ph.c4d3cv * vy; Fz = -ph.c2d3cv * vz; break; }
+        case T_PXY: Fx = ph.colvis * V(2); Fy = ph.colvis * V(1); Fz = 0.; break;
+        case T_PXZ: Fx = ph.colvis * V(3); Fy = 0.; Fz = ph.colvis * V(1); break;
+        case T_PYZ: Fx = 0.; Fy = ph.colvis * V(3); Fz = ph.colvis * V(2); break;
+        default: {
+            const double ct = V(5);
             Fx = f == T_QX ? ct : 0.;
             Fy = f == T_QY ? ct : 0.;
             Fz = f == T_QZ ? ct : 0.;
+        }
     }
 }
 
