@@ -800,23 +800,38 @@ __global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restric
     const int ni = g.nx + (DIR == 0), nj = g.ny + (DIR == 1);
     const long nfaces = (long)ni * nj * (g.nz + (DIR == 2));
     const long f0 = (long)xcd_block() * 64;  // XCD-contiguous ranges of faces
-    for (int it = (int)threadIdx.x; it < 2 * NQ * 64; it += NF * 64) {
-        const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
-        const long fid = f0 + l < nfaces ? f0 + l : nfaces - 1;
-        int ci = (int)(fid % ni), cj = (int)((fid / ni) % nj), ck = (int)(fid / ((long)ni * nj));
-        // side 0: the cell below the face (index a - 1, wrapped), side 1: above (index a, wrapped)
-        int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
-        const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
-        if (side == 0) a = a == 0 ? n_ax - 1 : a - 1;
-        else if (a == n_ax) a = 0;
-        const long cell = (long)ci + (long)g.nx * ((long)cj + (long)g.ny * ck);
-        double m1[1][NB], K1[1][NK];
+    // phase 1: NIT (side, field) items per thread, all their loads issued
+    // before the first is used
+    constexpr int NIT = (2 * NQ * 64 + NF * 64 - 1) / (NF * 64);
+    double m1[NIT][1][NB];
 #pragma unroll
-        for (int r = 0; r < NB; r++) m1[0][r] = q[((long)r * NQ + fld) * g.N + cell];
-        if (side == 0) face_coeffs<NB, 1, DIR, 1, NK>(m1, K1);
-        else face_coeffs<NB, 1, DIR, 0, NK>(m1, K1);
+    for (int u = 0; u < NIT; u++) {
+        const int it = (int)threadIdx.x + u * NF * 64;
+        if (it < 2 * NQ * 64) {
+            const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
+            const long fid = f0 + l < nfaces ? f0 + l : nfaces - 1;
+            int ci = (int)(fid % ni), cj = (int)((fid / ni) % nj), ck = (int)(fid / ((long)ni * nj));
+            // side 0: the cell below the face (index a - 1, wrapped), side 1: above (index a, wrapped)
+            int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
+            const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+            if (side == 0) a = a == 0 ? n_ax - 1 : a - 1;
+            else if (a == n_ax) a = 0;
+            const long cell = (long)ci + (long)g.nx * ((long)cj + (long)g.ny * ck);
 #pragma unroll
-        for (int k = 0; k < NK; k++) ks[side][fld][k][l] = K1[0][k];
+            for (int r = 0; r < NB; r++) m1[u][0][r] = q[((long)r * NQ + fld) * g.N + cell];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NIT; u++) {
+        const int it = (int)threadIdx.x + u * NF * 64;
+        if (it < 2 * NQ * 64) {
+            const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
+            double K1[1][NK];
+            if (side == 0) face_coeffs<NB, 1, DIR, 1, NK>(m1[u], K1);
+            else face_coeffs<NB, 1, DIR, 0, NK>(m1[u], K1);
+#pragma unroll
+            for (int k = 0; k < NK; k++) ks[side][fld][k][l] = K1[0][k];
+        }
     }
     __syncthreads();
     // lanes past the last face evaluate the last face again and store nothing
