@@ -642,11 +642,16 @@ def test_row_order_does_not_change_results(torch_mod):
 
 @pytest.mark.parametrize("name,env", [("tm13_nb4_roe_periodic", {"HD_FACES": "table"}),
                                       ("tgv_nb10_roe_8", {"HD_FACES": "table", "HD_LIMITER": "table"}),
-                                      ("tenmom_nb4_hllc_periodic", {"HD_LIMITER": "table"})])
+                                      ("tenmom_nb4_hllc_periodic", {"HD_LIMITER": "table"}),
+                                      ("vortex_nb10_hllc", {"HD_KERNELS": "generic"}),
+                                      ("relax_nq11_bodyforce", {"HD_KERNELS": "generic"}),
+                                      ("nb8_roe_walls", {"HD_KERNELS": "generic"})])
 def test_table_driven_variants(torch_mod, oracle_mod, name, env):
     """The one-thread-per-face k_face_flux (HD_FACES=table; periodic 13-moment
-    grids otherwise take fz::k_face13) and the table-driven limiter
-    (HD_LIMITER=table; fz::k_limit otherwise) against the oracle."""
+    grids otherwise take fz::k_face13), the table-driven limiter
+    (HD_LIMITER=table; fz::k_limit otherwise) and the table-driven face/cell
+    kernels for configurations the fused stage kernel runs
+    (HD_KERNELS=generic) against the oracle."""
     from hermeshd_amd.solver import Solver
     cfg = CASES[name]
     print(f"[table_variants] {name} {env}", flush=True)
