@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -239,14 +240,17 @@ def taylor_green_leg(n=192, steps=4, adj_n=48, adj_steps=20, checkpoint_every=5)
                         "ratio": round(tb / tf, 3), "KE": J, "norm_dKE_du0": gn}}
 
 
-def taylor_green_run(n, t_end, check_every=50, report_s=30.0):
+def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
     """The 13-moment Taylor-Green vortex (taylor_green_3d(n)) stepped with the
     reference's step semantics until t >= t_end.  One JSON line every
-    report_s seconds and at the end: time, steps, the mean kinetic energy of
-    the cell averages, its decay rate since the last report, and the drift of
-    the total mass, momentum and energy (relative to the totals of |.|).  The
-    decay rate at early times is compared with nu <|grad u|^2> = 3/(4 Re) of
-    the initial field (the incompressible Navier-Stokes dissipation rate)."""
+    report_s seconds of wall time (or, with report_t > 0, every report_t of
+    simulated time) and at the end: time, steps, the mean kinetic energy of
+    the cell averages (KE) and of the whole polynomial state by the 27-point
+    Gauss rule (KE_q), their decay rates since the last report, and the drift
+    of the total mass, momentum and energy (relative to the totals of |.|).
+    The decay rate at early times is compared with nu <|grad u|^2> = 3/(4 Re)
+    of the initial field (the incompressible Navier-Stokes dissipation
+    rate)."""
     import numpy as np
     import torch
     from hermeshd_amd import ics
@@ -259,16 +263,34 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0):
     q1 = torch.zeros_like(q)
     dV = (cfg.lx / cfg.nx) * (cfg.ly / cfg.ny) * (cfg.lz / cfg.nz)
 
+    # modes 1, x, y, z, yz, zx, xy, P2(x), P2(y), P2(z) at the 3-point Gauss
+    # nodes of each axis, weights normalised to 1 over the cell
+    gx, gw = (-math.sqrt(0.6), 0.0, math.sqrt(0.6)), (5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0)
+
+    def ke_quadrature():
+        m = q[:, :5]
+        acc = 0.0
+        for a, wa in zip(gx, gw):
+            for b, wb in zip(gx, gw):
+                for c, wc in zip(gx, gw):
+                    phi = (1.0, a, b, c, b * c, c * a, a * b, 1.5 * a * a - 0.5, 1.5 * b * b - 0.5,
+                           1.5 * c * c - 0.5)
+                    v = m[0] * phi[0]
+                    for r in range(1, cfg.nbasis):
+                        v = v + phi[r] * m[r]
+                    acc += wa * wb * wc / 8.0 * float((0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) / v[0]).mean())
+        return acc
+
     def diag():
         m = q[0]
         tot = (m[:5].reshape(5, -1).sum(1) * dV).cpu().numpy()
         ke = float((0.5 * (m[1] ** 2 + m[2] ** 2 + m[3] ** 2) / m[0]).mean())
-        return tot, ke
-    tot0, ke0 = diag()
+        return tot, ke, ke_quadrature()
+    tot0, ke0, keq0 = diag()
     scale = (q[0, :5].abs().reshape(5, -1).sum(1) * dV).cpu().numpy()
     eps0 = 0.75 / 1600.0
     log(f"tgv-run: {n}^3, t_end {t_end}, KE0 {ke0:.6f}, nu<|grad u|^2> {eps0:.4e}")
-    steps, t, t_last, ke_last = 0, 0.0, 0.0, ke0
+    steps, t, t_last, ke_last, keq_last = 0, 0.0, 0.0, ke0, keq0
     w0 = w_last = time.perf_counter()
     while t < t_end:
         for _ in range(check_every):
@@ -276,19 +298,21 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0):
         steps += check_every
         t = float(s.t)
         now = time.perf_counter()
-        if now - w_last >= report_s or t >= t_end:
+        due = (t - t_last >= report_t) if report_t > 0 else (now - w_last >= report_s)
+        if due or t >= t_end:
             s.check()
-            tot, ke = diag()
-            rec = {"n": n, "t": round(t, 6), "steps": steps, "KE": ke,
-                   "dKE_dt": (ke - ke_last) / (t - t_last), "rel_drift": ((tot - tot0) / scale).tolist(),
-                   "wall_s": round(now - w0, 1)}
+            tot, ke, keq = diag()
+            rec = {"n": n, "t": round(t, 6), "steps": steps, "KE": ke, "dKE_dt": (ke - ke_last) / (t - t_last),
+                   "KE_q": keq, "dKEq_dt": (keq - keq_last) / (t - t_last),
+                   "rel_drift": ((tot - tot0) / scale).tolist(), "wall_s": round(now - w0, 1)}
             print(json.dumps(rec), flush=True)
-            w_last, t_last, ke_last = now, t, ke
+            w_last, t_last, ke_last, keq_last = now, t, ke, keq
     s.check()
-    tot, ke = diag()
+    tot, ke, keq = diag()
     wall = time.perf_counter() - w0
     return {"workload": f"13-moment Taylor-Green vortex {n}^3, Re 1600, Pr 0.71, nbasis 10, Roe, limiter, SSP-RK3",
-            "t": t, "steps": steps, "KE0": ke0, "KE": ke, "mean_dKE_dt": (ke - ke0) / t, "nu_grad_u2_t0": eps0,
+            "t": t, "steps": steps, "KE0": ke0, "KE": ke, "KE_q0": keq0, "KE_q": keq,
+            "mean_dKE_dt": (ke - ke0) / t, "nu_grad_u2_t0": eps0,
             "max_rel_drift": float(np.max(np.abs((tot - tot0) / scale))), "ms_per_step": wall / steps * 1e3,
             "wall_s": round(wall, 1)}
 
@@ -362,6 +386,8 @@ def main():
     ap.add_argument("--tgv-run", type=int, default=0, metavar="N",
                     help="only run the 13-moment Taylor-Green vortex on N^3 cells to --tgv-t and print its diagnostics")
     ap.add_argument("--tgv-t", type=float, default=1.0)
+    ap.add_argument("--tgv-report-dt", type=float, default=0.0,
+                    help="report every this much simulated time (default: every 30 s of wall time)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -369,7 +395,8 @@ def main():
               "by slabs with one face-trace exchange per stage); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.tgv_run:
-        print(json.dumps({"tgv_run": taylor_green_run(args.tgv_run, args.tgv_t)}), flush=True)
+        print(json.dumps({"tgv_run": taylor_green_run(args.tgv_run, args.tgv_t, report_t=args.tgv_report_dt)}),
+              flush=True)
         return
 
     from hermeshd_amd import ics
