@@ -388,6 +388,8 @@ def main():
     ap.add_argument("--tgv-t", type=float, default=1.0)
     ap.add_argument("--tgv-report-dt", type=float, default=0.0,
                     help="report every this much simulated time (default: every 30 s of wall time)")
+    ap.add_argument("--tgv-adj", type=int, nargs=3, default=(48, 20, 5), metavar=("N", "STEPS", "EVERY"),
+                    help="Taylor-Green leg's adjoint: N^3 cells, STEPS steps, a checkpoint every EVERY steps")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -507,7 +509,8 @@ def main():
         log(f"bench: adjoint {out['adjoint']}")
     if not args.no_tgv:
         log("bench: 13-moment Taylor-Green leg...")
-        out["taylor_green"] = taylor_green_leg()
+        an, asteps, aevery = args.tgv_adj
+        out["taylor_green"] = taylor_green_leg(adj_n=an, adj_steps=asteps, checkpoint_every=aevery)
         log(f"bench: taylor_green {out['taylor_green']}")
     if not args.no_ensemble:
         log("bench: ensemble leg (256 realizations)...")
