@@ -928,6 +928,25 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
     double mf[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) mf[r] = qin[qoff(g, cc, f, r)];
+    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+    // face lift of field f first, its loads beside those of the modes (one
+    // memory phase instead of two), one direction at a time:
+    // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
+    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
+    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
+    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+    const long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz,
+               nfz = (long)g.nx * g.ny * (g.nz + 1);
+    double t[NB];
+    if constexpr (MOM) {
+        fz::lift13_mom<NB, 0>(flx, nfx, fxm, fxm + 1, dxi, f, t);
+        fz::lift13_mom<NB, 1>(fly, nfy, fym, fym + g.nx, dyi, f, t);
+        fz::lift13_mom<NB, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, t);
+    } else {
+        lift13_dir<NB, NF, 0>(flx, nfx, fxm, fxm + 1, dxi, f, T, t);
+        lift13_dir<NB, NF, 1>(fly, nfy, fym, fym + g.nx, dyi, f, T, t);
+        lift13_dir<NB, NF, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
+    }
     // ---- volume integral: every wave reaches every barrier ----
     double acc[NS];
 #pragma unroll
@@ -987,7 +1006,6 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
         __syncthreads();  // the next plane overwrites qp_s and pr_s
     }
     if (!live) return;  // no barrier below
-    const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
     double intr[NB];
     intr[0] = 0.;
     intr[KX] = 0.25 * T->cb[KX] * dxi * acc[0];
@@ -1004,23 +1022,6 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
         } else {
             intr[KXYZ] = 0.25 * T->cb[KXYZ] * (dxi * acc[9] + dyi * acc[10] + dzi * acc[11]);
         }
-    }
-    // face lift of field f, one direction at a time:
-    // t[r] = dxi*sx[r] + dyi*sy[r] + dzi*sz[r] with the sums of k_cell
-    const long fxm = (long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k);
-    const long fym = (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k);
-    const long fzm = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
-    const long nfx = (long)(g.nx + 1) * g.ny * g.nz, nfy = (long)g.nx * (g.ny + 1) * g.nz,
-               nfz = (long)g.nx * g.ny * (g.nz + 1);
-    double t[NB];
-    if constexpr (MOM) {
-        fz::lift13_mom<NB, 0>(flx, nfx, fxm, fxm + 1, dxi, f, t);
-        fz::lift13_mom<NB, 1>(fly, nfy, fym, fym + g.nx, dyi, f, t);
-        fz::lift13_mom<NB, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, t);
-    } else {
-        lift13_dir<NB, NF, 0>(flx, nfx, fxm, fxm + 1, dxi, f, T, t);
-        lift13_dir<NB, NF, 1>(fly, nfy, fym, fym + g.nx, dyi, f, T, t);
-        lift13_dir<NB, NF, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
     }
     double glf[NB];
     glf[0] = 0.25 * t[0];
