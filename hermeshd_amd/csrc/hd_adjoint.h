@@ -970,15 +970,15 @@ __device__ __forceinline__ void block_partial13(double v, double* partial) {
     }
 }
 
-// One field per wave over 64 consecutive cells (as k_cell13).  The volume
-// transpose takes one row of the rule (QP points) per exchange: each wave
-// writes its field's point value and the cotangents of its three point
-// fluxes (the transposes of k_cell's volume sums); one wave per point then
-// reverses the point's physical fluxes (flux_vjp) and shares the state
-// cotangent, which each wave projects onto its field's modes.  The faces add
-// bfc^T of the cotangents written by k_face13_vjp.  Then the sources, the
-// stage map (lo + dt*rt, the accumulated terms) and the frozen limiter, as
-// cell_vjp_finish; d/dcoll as cell_vjp_parts.
+// One field per wave over 64 consecutive cells (as k_cell13).  The faces
+// come first (their loads beside the modes'): bfc^T of the cotangents written
+// by k_face13_vjp.  The volume transpose then takes one row of the rule (QP
+// points) per exchange: each wave writes its field's point value and the
+// cotangents of its three point fluxes (the transposes of k_cell's volume
+// sums); one wave per point reverses the point's physical fluxes (flux_vjp)
+// and shares the state cotangent, which each wave projects onto its field's
+// modes.  Then the sources, the stage map (lo + dt*rt, the accumulated terms)
+// and the frozen limiter, as cell_vjp_finish; d/dcoll as cell_vjp_parts.
 template <int NB, int QP, int RM>
 __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13_vjp(
     const double* __restrict__ q, const double* __restrict__ lam, double* __restrict__ lam_in,
@@ -1007,6 +1007,29 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13_vjp(
         mf[r] = q[o];
         lof[r] = ap.b * lam[o] * fac;
         rt[r] = 0.;
+    }
+    // ---- faces first, their loads beside those of the modes: the cell is
+    // the right side of its low faces and the left side of its high faces ----
+    {
+        const long nfl[3] = {(long)(g.nx + 1) * g.ny * g.nz, (long)g.nx * (g.ny + 1) * g.nz,
+                             (long)g.nx * g.ny * (g.nz + 1)};
+        const long fm[3] = {(long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k),
+                            (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k),
+                            (long)i + (long)g.nx * ((long)j + (long)g.ny * k)};
+        const long fp[3] = {fm[0] + 1, fm[1] + g.nx, fm[2] + (long)g.nx * g.ny};
+        const double* const bqr[3] = {bqr0, bqr1, bqr2};
+        const double* const bql[3] = {bql0, bql1, bql2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll 1
+            for (int p = 0; p < NF; p++) {
+                const long o = (long)f * NF + p;
+                const double vR = bqr[d][o * nfl[d] + fm[d]];  // this cell is the right side of its low face
+                const double vL = bql[d][o * nfl[d] + fp[d]];  // and the left side of its high face
+#pragma unroll
+                for (int r = 0; r < NB; r++) rt[r] += T->bfc[2 * d][p][r] * vR + T->bfc[2 * d + 1][p][r] * vL;
+            }
+        }
     }
     // ---- volume transpose; every wave reaches every barrier ----
 #pragma unroll 1
@@ -1074,27 +1097,6 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13_vjp(
     }
     double dcoll = 0.;
     if (live) {
-        // ---- faces: the cell is the right side of its low faces and the
-        // left side of its high faces ----
-        const long nfl[3] = {(long)(g.nx + 1) * g.ny * g.nz, (long)g.nx * (g.ny + 1) * g.nz,
-                             (long)g.nx * g.ny * (g.nz + 1)};
-        const long fm[3] = {(long)i + (long)(g.nx + 1) * ((long)j + (long)g.ny * k),
-                            (long)i + (long)g.nx * ((long)j + (long)(g.ny + 1) * k),
-                            (long)i + (long)g.nx * ((long)j + (long)g.ny * k)};
-        const long fp[3] = {fm[0] + 1, fm[1] + g.nx, fm[2] + (long)g.nx * g.ny};
-        const double* const bqr[3] = {bqr0, bqr1, bqr2};
-        const double* const bql[3] = {bql0, bql1, bql2};
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-#pragma unroll 1
-            for (int p = 0; p < NF; p++) {
-                const long o = (long)f * NF + p;
-                const double vR = bqr[d][o * nfl[d] + fm[d]];  // this cell is the right side of its low face
-                const double vL = bql[d][o * nfl[d] + fp[d]];  // and the left side of its high face
-#pragma unroll
-                for (int r = 0; r < NB; r++) rt[r] += T->bfc[2 * d][p][r] * vR + T->bfc[2 * d + 1][p][r] * vL;
-            }
-        }
         // ---- sources (as cell_vjp_parts) ----
         if (ap.ivis == 0) {
             if (f >= PXX && f <= T_PYZ && ap.coll != 0.)
