@@ -119,9 +119,9 @@ int hd_min_dt_static(const hd_config *cfg, const double *q, double *dt_dev, void
    the stage kernels.  On return q_io holds the new state; q1 and q2 hold
    unspecified values.  With noise on, one stochastic stress field is drawn
    for the step (key seed, realization, step index; amplitude from *dt_dev;
-   computed in FP64, stored in FP32, the reference's precision for its
-   Gaussian draws), used by every stage, and the step index then advances by
-   one. */
+   Gaussian draws by an FP32 Box-Muller transform, the reference's precision
+   for them, combined into the stress in FP64 and stored in FP32), used by
+   every stage, and the step index then advances by one. */
 int hd_update(hd_ctx *ctx, double *q_io, double *q1, double *q2, const double *dt_dev, void *stream);
 
 /* dt = min_dt(q_io); update; t += dt.  t_dev and dt_dev are device doubles.
@@ -142,8 +142,11 @@ int hd_stage(hd_ctx *ctx, int32_t rk_order, int32_t stage_index, double *q_in, c
 /* ---- adjoint (reverse mode) ------------------------------------------------
    No counterpart in the reference, which has no adjoint: these give the
    gradient of a run with respect to its initial state and the relaxation
-   rate coll (used for the Poiseuille-channel sensitivity).  Central and HLLC
-   face solvers; no noise. */
+   rate coll (used for the Poiseuille-channel sensitivity).  Central, HLLC and
+   Roe face solvers; no noise.  On a 13-moment context periodic in every
+   direction, the first hd_rhs_vjp or hd_stage_vjp allocates the face
+   cotangents of the split transpose (3 arrays of nQ*iquad^2 doubles per face,
+   counted by hd_device_bytes) unless HD_VJP=fused was set at hd_create. */
 
 /* out = (dR/dq)^T lam, R the right-hand side of hd_rhs at state q. */
 int hd_rhs_vjp(hd_ctx *ctx, const double *q, const double *lam, double *out, void *stream);
