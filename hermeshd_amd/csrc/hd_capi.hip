@@ -259,7 +259,9 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
                  nfz = (size_t)g.nx * g.ny * (g.nz + 1);
     c->npart = 1024;
     // the full 10-moment equations (ivis 2) and the water EOS (ieos 2) run on the
-    // table-driven kernels only; HD_KERNELS=generic selects them for any configuration
+    // table-driven kernels only, and the 13-moment model on the same face/cell
+    // split with its own kernels; HD_KERNELS=generic selects the table-driven
+    // kernels for any configuration
     const char* kern = std::getenv("HD_KERNELS");
     c->generic = (kern && std::strcmp(kern, "generic") == 0) || cfg->ivis == 2 || cfg->ieos == 2 ||
                  cfg->nQ == 13;
@@ -267,8 +269,8 @@ int hd_create(const hd_config* cfg, hd_ctx** out) {
         hd_destroy(c);
         return fail(HD_ERR_UNSUPPORTED, "the table-driven kernels (HD_KERNELS=generic) carry no stochastic stress");
     }
-    // face-flux arrays: only the table-driven kernels write them (the fused
-    // stage kernel keeps its face fluxes in registers and LDS)
+    // face-flux arrays: only the face/cell split writes them (the fused stage
+    // kernel keeps its face fluxes in registers and LDS)
     if ((r = alloc(c, (void**)&c->dtab, sizeof(Tables))) ||
         (c->generic && ((r = alloc(c, (void**)&c->fl[0], nf * nfx * sizeof(double))) ||
                         (r = alloc(c, (void**)&c->fl[1], nf * nfy * sizeof(double))) ||

@@ -134,8 +134,8 @@ struct Impl {
         }
     }
     static Ops ops() {
-        // the 13-moment model runs forward on the table-driven kernels (its
-        // transpose is the fused adjoint kernel); no ensembles
+        // the 13-moment model runs on its own face and cell kernels (no fused
+        // stage, no ensembles); its transpose is split on periodic grids
         if constexpr (NQ == 13) return Ops{&limiter, &faces, &cell, nullptr, nullptr, nullptr, &stage_vjp};
         else return Ops{&limiter, &faces, &cell, &fused, &ens_limit, &ens_stage, &stage_vjp};
     }
