@@ -240,6 +240,26 @@ def taylor_green_leg(n=192, steps=4, adj_n=48, adj_steps=20, checkpoint_every=5)
                         "ratio": round(tb / tf, 3), "KE": J, "norm_dKE_du0": gn}}
 
 
+def kinetic_energy_quadrature(q, nbasis):
+    """Mean over the cells of 0.5 |m|^2 / rho of the whole polynomial state,
+    by the 27-point Gauss rule (exact for the squares of complete-quadratic
+    fields at constant density).  q: (nbasis, nQ, nz, ny, nx) tensor, modes
+    1, x, y, z, yz, zx, xy, then xyz (nbasis 8) or P2(x), P2(y), P2(z)."""
+    gx, gw = (-math.sqrt(0.6), 0.0, math.sqrt(0.6)), (5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0)
+    m = q[:, :5]
+    acc = 0.0
+    for a, wa in zip(gx, gw):
+        for b, wb in zip(gx, gw):
+            for c, wc in zip(gx, gw):
+                phi = [1.0, a, b, c, b * c, c * a, a * b]
+                phi += [a * b * c] if nbasis == 8 else [1.5 * a * a - 0.5, 1.5 * b * b - 0.5, 1.5 * c * c - 0.5]
+                v = m[0] * phi[0]
+                for r in range(1, nbasis):
+                    v = v + phi[r] * m[r]
+                acc += wa * wb * wc / 8.0 * float((0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) / v[0]).mean())
+    return acc
+
+
 def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
     """The 13-moment Taylor-Green vortex (taylor_green_3d(n)) stepped with the
     reference's step semantics until t >= t_end.  One JSON line every
@@ -263,29 +283,11 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
     q1 = torch.zeros_like(q)
     dV = (cfg.lx / cfg.nx) * (cfg.ly / cfg.ny) * (cfg.lz / cfg.nz)
 
-    # modes 1, x, y, z, yz, zx, xy, P2(x), P2(y), P2(z) at the 3-point Gauss
-    # nodes of each axis, weights normalised to 1 over the cell
-    gx, gw = (-math.sqrt(0.6), 0.0, math.sqrt(0.6)), (5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0)
-
-    def ke_quadrature():
-        m = q[:, :5]
-        acc = 0.0
-        for a, wa in zip(gx, gw):
-            for b, wb in zip(gx, gw):
-                for c, wc in zip(gx, gw):
-                    phi = (1.0, a, b, c, b * c, c * a, a * b, 1.5 * a * a - 0.5, 1.5 * b * b - 0.5,
-                           1.5 * c * c - 0.5)
-                    v = m[0] * phi[0]
-                    for r in range(1, cfg.nbasis):
-                        v = v + phi[r] * m[r]
-                    acc += wa * wb * wc / 8.0 * float((0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) / v[0]).mean())
-        return acc
-
     def diag():
         m = q[0]
         tot = (m[:5].reshape(5, -1).sum(1) * dV).cpu().numpy()
         ke = float((0.5 * (m[1] ** 2 + m[2] ** 2 + m[3] ** 2) / m[0]).mean())
-        return tot, ke, ke_quadrature()
+        return tot, ke, kinetic_energy_quadrature(q, cfg.nbasis)
     tot0, ke0, keq0 = diag()
     scale = (q[0, :5].abs().reshape(5, -1).sum(1) * dV).cpu().numpy()
     eps0 = 0.75 / 1600.0
