@@ -948,9 +948,34 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
         lift13_dir<NB, NF, 2>(flz, nfz, fzm, fzm + (long)g.nx * g.ny, dzi, f, T, t);
     }
     // ---- volume integral: every wave reaches every barrier ----
+    // glf[r] = 0.25 cb[r] (t[r] - sum_d di acc_d[r]) for r >= 1, so each
+    // accumulator starts at -t[r] over its grid factor and t[1..] need not
+    // stay live through the volume loop (the kernel runs at 128 VGPRs)
     double acc[NS];
-#pragma unroll
-    for (int s = 0; s < NS; s++) acc[s] = 0.;
+    {
+        const double dx = 1. / dxi, dy = 1. / dyi, dz = 1. / dzi;
+        acc[0] = -dx * t[KX];
+        acc[1] = -dy * t[KY];
+        acc[2] = -dz * t[KZ];
+        if constexpr (NB > 4) {
+            acc[3] = -dy * t[KYZ];
+            acc[4] = 0.;
+            acc[5] = -dx * t[KZX];
+            acc[6] = 0.;
+            acc[7] = -dx * t[KXY];
+            acc[8] = 0.;
+            if constexpr (NB == 10) {
+                acc[9] = -dx * t[KXX];
+                acc[10] = -dy * t[KYY];
+                acc[11] = -dz * t[KZZ];
+            } else {
+                acc[9] = -dx * t[KXYZ];
+                acc[10] = 0.;
+                acc[11] = 0.;
+            }
+        }
+    }
+    const double t0 = t[0];
     // volume integral, NCH points per exchange: each wave writes its field's
     // point values, then reads the 13 values of each point; every wave
     // reaches every barrier
@@ -1024,9 +1049,9 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
         }
     }
     double glf[NB];
-    glf[0] = 0.25 * t[0];
+    glf[0] = 0.25 * t0;
 #pragma unroll
-    for (int r = 1; r < NB; r++) glf[r] = 0.25 * T->cb[r] * t[r] - intr[r];
+    for (int r = 1; r < NB; r++) glf[r] = -intr[r];
     // ---- source: explicit relaxation (ivis 0), body force ----
     double src[NB];
 #pragma unroll
