@@ -48,6 +48,9 @@ int validate(const hd_config* c) {
     if (c->ivis == 2 && c->nQ != 11) return fail(HD_ERR_INVALID, "ivis=2 needs the second moments (nQ=11)");
     if (c->nQ == 13 && (c->noise || c->ieos != 1))
         return fail(HD_ERR_INVALID, "the 13-moment model runs with an ideal gas and no noise");
+    // the 13-moment kernels index cells and faces in 32 bits
+    if (c->nQ == 13 && (long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1) > 2147483647L)
+        return fail(HD_ERR_INVALID, "the 13-moment model takes at most 2^31 - 1 faces per direction");
     if (c->ivis == 2 && c->noise) return fail(HD_ERR_INVALID, "ivis=2 has no stochastic stress");
     if (c->ieos != 1 && c->ieos != 2) return fail(HD_ERR_INVALID, "ieos must be 1 (ideal gas) or 2 (water)");
     if (c->ieos == 2 && c->riemann == HD_RIEMANN_ROE)

@@ -803,20 +803,31 @@ __global__ __launch_bounds__(QP * QP * 64) void k_face13(const double* __restric
     // phase 1: NIT (side, field) items per thread, all their loads issued
     // before the first is used
     constexpr int NIT = (2 * NQ * 64 + NF * 64 - 1) / (NF * 64);
+    // the item stride NF*64 is a multiple of 64, so every item of a thread
+    // belongs to its lane's face: the two cells are found once per lane, in
+    // 32-bit arithmetic (hd_create bounds the faces per direction by 2^31)
+    long cell2[2];
+    {
+        const long fl64 = f0 + (long)(threadIdx.x & 63);
+        const unsigned fid = (unsigned)(fl64 < nfaces ? fl64 : nfaces - 1);
+        const unsigned fq = fid / (unsigned)ni;
+        int ci = (int)(fid - fq * (unsigned)ni), cj = (int)(fq % (unsigned)nj), ck = (int)(fq / (unsigned)nj);
+        int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
+        const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+        // side 0: the cell below the face (index a - 1, wrapped), side 1: above (index a, wrapped)
+        const int a_lo = a == 0 ? n_ax - 1 : a - 1, a_hi = a == n_ax ? 0 : a;
+        a = a_lo;
+        cell2[0] = (long)(unsigned)(ci + g.nx * (cj + g.ny * ck));
+        a = a_hi;
+        cell2[1] = (long)(unsigned)(ci + g.nx * (cj + g.ny * ck));
+    }
     double m1[NIT][1][NB];
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
         const int it = (int)threadIdx.x + u * NF * 64;
         if (it < 2 * NQ * 64) {
-            const int l = it & 63, side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
-            const long fid = f0 + l < nfaces ? f0 + l : nfaces - 1;
-            int ci = (int)(fid % ni), cj = (int)((fid / ni) % nj), ck = (int)(fid / ((long)ni * nj));
-            // side 0: the cell below the face (index a - 1, wrapped), side 1: above (index a, wrapped)
-            int& a = DIR == 0 ? ci : DIR == 1 ? cj : ck;
-            const int n_ax = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
-            if (side == 0) a = a == 0 ? n_ax - 1 : a - 1;
-            else if (a == n_ax) a = 0;
-            const long cell = (long)ci + (long)g.nx * ((long)cj + (long)g.ny * ck);
+            const int side = (it >> 6) / NQ, fld = (it >> 6) % NQ;
+            const long cell = side == 0 ? cell2[0] : cell2[1];
 #pragma unroll
             for (int r = 0; r < NB; r++) m1[u][0][r] = q[((long)r * NQ + fld) * g.N + cell];
         }
@@ -924,7 +935,9 @@ __global__ __launch_bounds__(CELL13_WAVES * 64) void k_cell13(
     const long c = (long)xcd_block() * 64 + lane;  // XCD-contiguous ranges of cells
     const bool live = c < g.N;
     const long cc = live ? c : g.N - 1;  // lanes past the grid compute a copy of the last cell and store nothing
-    const int i = (int)(cc % g.nx), j = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long)g.nx * g.ny));
+    // 32-bit position arithmetic: hd_create bounds the cells by 2^31
+    const unsigned ucc = (unsigned)cc, cq = ucc / (unsigned)g.nx;
+    const int i = (int)(ucc - cq * (unsigned)g.nx), j = (int)(cq % (unsigned)g.ny), k = (int)(cq / (unsigned)g.ny);
     double mf[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) mf[r] = qin[qoff(g, cc, f, r)];
