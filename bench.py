@@ -25,6 +25,7 @@ import os
 import sys
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -260,7 +261,7 @@ def kinetic_energy_quadrature(q, nbasis):
     return acc
 
 
-def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
+def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0, wall_limit_s=0.0):
     """The 13-moment Taylor-Green vortex (taylor_green_3d(n)) stepped with the
     reference's step semantics until t >= t_end.  One JSON line every
     report_s seconds of wall time (or, with report_t > 0, every report_t of
@@ -270,7 +271,8 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
     of the total mass, momentum and energy (relative to the totals of |.|).
     The decay rate at early times is compared with nu <|grad u|^2> = 3/(4 Re)
     of the initial field (the incompressible Navier-Stokes dissipation
-    rate)."""
+    rate).  With wall_limit_s > 0 the run also stops once that many seconds
+    have passed since the process started, and says so in its last line."""
     import numpy as np
     import torch
     from hermeshd_amd import ics
@@ -294,7 +296,11 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
     log(f"tgv-run: {n}^3, t_end {t_end}, KE0 {ke0:.6f}, nu<|grad u|^2> {eps0:.4e}")
     steps, t, t_last, ke_last, keq_last = 0, 0.0, 0.0, ke0, keq0
     w0 = w_last = time.perf_counter()
+    stopped = False
     while t < t_end:
+        if wall_limit_s > 0 and time.perf_counter() - T_START > wall_limit_s:
+            stopped = True
+            break
         for _ in range(check_every):
             s.step(q, q1)
         steps += check_every
@@ -316,7 +322,7 @@ def taylor_green_run(n, t_end, check_every=50, report_s=30.0, report_t=0.0):
             "t": t, "steps": steps, "KE0": ke0, "KE": ke, "KE_q0": keq0, "KE_q": keq,
             "mean_dKE_dt": (ke - ke0) / t, "nu_grad_u2_t0": eps0,
             "max_rel_drift": float(np.max(np.abs((tot - tot0) / scale))), "ms_per_step": wall / steps * 1e3,
-            "wall_s": round(wall, 1)}
+            "wall_s": round(wall, 1), "setup_s": round(w0 - T_START, 1), "stopped_at_wall_limit": stopped}
 
 
 def ensemble_leg(steps=20):
@@ -390,6 +396,8 @@ def main():
     ap.add_argument("--tgv-t", type=float, default=1.0)
     ap.add_argument("--tgv-report-dt", type=float, default=0.0,
                     help="report every this much simulated time (default: every 30 s of wall time)")
+    ap.add_argument("--tgv-wall", type=float, default=0.0, metavar="S",
+                    help="stop the --tgv-run loop S seconds after the process started (0: no limit)")
     ap.add_argument("--tgv-adj", type=int, nargs=3, default=(48, 20, 5), metavar=("N", "STEPS", "EVERY"),
                     help="Taylor-Green leg's adjoint: N^3 cells, STEPS steps, a checkpoint every EVERY steps")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -399,8 +407,8 @@ def main():
               "by slabs with one face-trace exchange per stage); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.tgv_run:
-        print(json.dumps({"tgv_run": taylor_green_run(args.tgv_run, args.tgv_t, report_t=args.tgv_report_dt)}),
-              flush=True)
+        print(json.dumps({"tgv_run": taylor_green_run(args.tgv_run, args.tgv_t, report_t=args.tgv_report_dt,
+                                                      wall_limit_s=args.tgv_wall)}), flush=True)
         return
 
     from hermeshd_amd import ics
