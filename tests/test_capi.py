@@ -86,3 +86,20 @@ def test_min_dt_static_refuses_large_grids_before_any_device_call():
     assert L.hd_min_dt_static(ctypes.byref(cs), dummy, dummy, None) == _lib.HD_ERR_INVALID
     assert b"16777216" in L.hd_last_error()
     assert L.hd_min_dt_static(None, dummy, dummy, None) == _lib.HD_ERR_INVALID
+
+
+def test_create_refuses_13_moment_grids_past_32_bit_face_indices():
+    """The 13-moment kernels index cells and faces in 32 bits, so hd_create
+    refuses a grid with (nx+1)(ny+1)(nz+1), a bound on the faces of every
+    direction, above 2^31 - 1; the check runs before any HIP call (so it runs
+    here without a GPU).  1290^3 cells give 1291^3 = 2.152e9."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    from hermeshd_amd.config import taylor_green_3d
+    L = _lib.lib()
+    cfg = taylor_green_3d(1290)
+    assert 1291 * 1290 * 1290 > 2 ** 31 - 1
+    cs = _lib.config_struct(cfg.as_ints())
+    out = ctypes.c_void_p()
+    assert L.hd_create(ctypes.byref(cs), ctypes.byref(out)) == _lib.HD_ERR_INVALID
+    assert b"2^31" in L.hd_last_error() and not out.value
