@@ -77,7 +77,9 @@ typedef struct hd_config {
 typedef struct hd_ctx hd_ctx;
 
 /* Builds basis/quadrature tables, geometry and scratch on the current device.
-   Replaces initializer (src/initialize.f90:24-61) minus MPI and output. */
+   Replaces initializer (src/initialize.f90:24-61) minus MPI and output.
+   HD_ERR_INVALID for an invalid config, checked before any device call; for
+   nQ 13 that includes (nx+1)(ny+1)(nz+1) > 2^31 - 1 (32-bit face indices). */
 int hd_create(const hd_config *cfg, hd_ctx **out);
 int hd_destroy(hd_ctx *ctx);
 
